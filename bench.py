@@ -1,0 +1,181 @@
+"""Benchmark: Mrays/s of the path tracer at 1080p on one MI355X.
+
+Workload (default): the instanced-mesh scene (16 base meshes, 4096 instances, ~26M instanced
+triangles, seed 0) at 1920x1080, 12 bounces, contribution factor 4, one sample per pixel per step
+(one reference dispatch, pathtracer.rs:225-239), 64 steps = the 64-spp image.  A ray is one
+intersect_scene call (primary or continuation).  Inputs are resident in HBM before timing.
+
+Prints one JSON line with the metric, a roofline object for the render kernel (algorithmic bytes
+per launch / average launch time from HIP events on the render stream) and a CPU baseline (the
+C oracle on a bounded band of rows of the same frame, OpenMP threads as stated).
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK_HBM_GBS = 8000.0
+
+
+def algorithmic_bytes(c):
+    """Bytes the traversal and shading must read/write: 32 per box test, 48 per triangle test,
+    64 per instance entry, 304 per hit (3 x 48-B vertices + 160-B instance), 64 per miss (four
+    RGBA32F texels), 32 per ray (two Sobol entries), 48 per sample (jitter entry + RGBA32F
+    read-modify-write)."""
+    misses = c["rays"] - c["hits"]
+    return (32 * c["n_aabb"] + 48 * c["n_tri"] + 64 * c["n_inst"] + 304 * c["hits"] + 64 * misses
+            + 32 * c["rays"] + 48 * c["samples"])
+
+
+def build_workload(name, share_blas=True):
+    from nbounce_amd import scenegen
+    import nbounce_amd as nb
+    if name == "large":
+        P = scenegen.LARGE
+        v, i, pr = scenegen.instanced_meshes(0)
+        desc = "instanced meshes: 16 base meshes, 4096 instances"
+    else:
+        P = scenegen.SMALL
+        v, i, pr = scenegen.spheres_in_box(0)
+        desc = "64 icospheres in an open box"
+    scene = nb.Scene(v, i, pr)
+    blas, tlas, inst = nb.build_scene(scene, share_blas=share_blas)
+    n_inst_tris = int(((pr["index_end"] - pr["index_start"]) // 3).sum())
+    return P, desc, scene, (blas, tlas, inst), n_inst_tris
+
+
+def cpu_baseline(P, scene, arrays, env_faces, cam, table, frame=1, target_s=15.0):
+    """Times the C oracle (the CPU restatement of the reference shader) on a band of rows."""
+    import oracle
+    blas, tlas, inst = arrays
+    osc = oracle.Scene(blas, tlas, inst, scene.vertices, scene.indices)
+    oenv = oracle.Env(env_faces)
+    threads = max(1, min(16, os.cpu_count() or 1))
+    W, H = P["width"], P["height"]
+    g = oracle.globals_(frame, 1.0, P["bounces"], P["contribution_factor"])
+    rows = 8
+    y0 = H // 2 - rows // 2
+    t = time.perf_counter()
+    _, cnt = oracle.render(osc, cam, g, table, 17, oenv, W, H, rect=(0, y0, W, y0 + rows), threads=threads)
+    dt = time.perf_counter() - t
+    rows = int(min(H, max(8, rows * target_s / max(dt, 1e-3))))
+    y0 = max(0, H // 2 - rows // 2)
+    t = time.perf_counter()
+    _, cnt = oracle.render(osc, cam, g, table, 17, oenv, W, H, rect=(0, y0, W, y0 + rows), threads=threads)
+    dt = time.perf_counter() - t
+    return {"value": round(cnt["rays"] / dt / 1e6, 4), "unit": "Mrays/s", "cores": threads, "kind": "port",
+            "sample": f"C oracle, frame {frame}, rows {y0}..{y0 + rows} of {W}x{H} ({int(cnt['rays'])} rays, "
+                      f"{dt:.1f} s, {threads} OpenMP threads)"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=64)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", choices=["large", "small"], default="large")
+    ap.add_argument("--frames-per-launch", type=int, default=1,
+                    help="frames per kernel launch (nb_render_frames); 1 = one nb_render per step")
+    ap.add_argument("--reference-blas", action="store_true", help="rebuild the BLAS per instance as the reference")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported "
+              "(the path shards by image tiles or Sobol rows, which this build does not run)", file=sys.stderr)
+        sys.exit(2)
+
+    import nbounce_amd as nb
+    from nbounce_amd import scenegen
+
+    P, desc, scene, arrays, n_inst_tris = build_workload(args.workload, share_blas=not args.reference_blas)
+    blas, tlas, inst = arrays
+    ctx = nb.Context(0)
+    sb = nb.SceneBuffers(ctx, blas, tlas, inst, scene.vertices, scene.indices)
+    env_faces = scenegen.sky_cube(P["env_face"])
+    env = nb.EnvMap(ctx, env_faces)
+    cam = nb.CameraController(world_position=P["eye"], target=P["target"], aspect_ratio=P["width"] / P["height"])
+    pt = nb.Pathtracer(ctx, sb, cam, env, P["width"], P["height"], resolution_factor=1.0)
+    pt.globals.bounces = P["bounces"]
+    pt.globals.contribution_factor = P["contribution_factor"]
+    F = max(1, args.frames_per_launch)
+
+    def step_block(n_steps):
+        done = 0
+        while done < n_steps:
+            if F == 1:
+                pt.dispatch()
+                done += 1
+            else:
+                done += pt.dispatch_frames(min(F, n_steps - done))
+
+    step_block(args.warmup)
+    ctx.synchronize()
+    pt.invalidate()
+    ctx.reset_counters()
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    ctx.timer_start()
+    step_block(args.steps)
+    dev_ms = ctx.timer_stop()
+    ctx.synchronize()
+    wall = time.perf_counter() - t0
+    c = ctx.counters()
+    launches = args.steps if F == 1 else math.ceil(args.steps / F)
+    rays = c["rays"]
+    value = rays / wall / 1e6
+    kernel_ms = dev_ms / launches
+    bytes_per_launch = algorithmic_bytes(c) / launches
+    achieved = bytes_per_launch / (kernel_ms * 1e-3) / 1e9
+
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(P, scene, arrays, env_faces, pt._camera_data, pt.table, frame=1, target_s=args.cpu_seconds)
+
+    out = {
+        "metric": "Mrays/s at 1080p" if args.workload == "large" else "Mrays/s at 256x256",
+        "value": round(value, 3),
+        "unit": "Mrays/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(wall * 1e3 / args.steps, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded scene generator, seed 0)",
+        "config": {
+            "workload": f"{desc}, {n_inst_tris} instanced triangles, {P['width']}x{P['height']}, "
+                        f"{args.steps} spp (1 spp per step), {P['bounces']} bounces",
+            "blas": "per instance (reference)" if args.reference_blas else "shared per mesh",
+            "frames_per_launch": F,
+            "rays": int(rays),
+            "rays_per_sample": round(rays / max(1, c["samples"]), 4),
+            "samples_per_s_M": round(c["samples"] / wall / 1e6, 3),
+        },
+        "roofline": {
+            "bound": "hbm",
+            "achieved": round(achieved, 2),
+            "peak": PEAK_HBM_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved / PEAK_HBM_GBS, 5),
+            "traffic": None,
+            "kernel": "render_kernel",
+            "kernel_ms": round(kernel_ms, 4),
+            "algorithmic_bytes_per_launch": int(bytes_per_launch),
+        },
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

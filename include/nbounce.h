@@ -1,0 +1,178 @@
+/*
+ * nbounce.h — C ABI of the MI355X (gfx950) path-tracing library.
+ *
+ * It replaces the renderer surface of the reference's `pathtracing` module.  Each entry
+ * point names the reference interface it stands in for (paths relative to the reference
+ * repository).  All structs below are byte-for-byte the reference's bytemuck layouts, so a
+ * Rust caller passes `bytemuck::cast_slice` bytes straight through.
+ *
+ * Conventions
+ *   - Every call returns nb_status; NB_OK == 0.  On failure nb_last_error() returns a
+ *     thread-local message.  No call aborts the process.
+ *   - Host arrays stay owned by the caller; the library copies what it needs during the call.
+ *   - A context is externally synchronized (one host thread at a time).  Rendering calls are
+ *     asynchronous on the context's HIP stream; nb_target_read() and nb_context_synchronize()
+ *     wait for them.
+ *   - Destroy functions accept NULL.
+ */
+#ifndef NBOUNCE_H
+#define NBOUNCE_H
+
+#include <stdint.h>
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+    NB_OK = 0,
+    NB_ERR_INVALID_ARGUMENT = 1,
+    NB_ERR_OUT_OF_MEMORY = 2,
+    NB_ERR_HIP = 3,
+    NB_ERR_BAD_LAYOUT = 4,
+    NB_ERR_UNSUPPORTED = 5,
+    NB_ERR_BUDGET = 6
+} nb_status;
+
+/* ---- reference byte layouts ---------------------------------------------------------- */
+
+/* BVHNode, src/pathtracing/bvh.rs:15-24 (32 B). Leaf <=> end > 0, range [start, end);
+ * inner: end == 0, children start and start + 1. */
+typedef struct { float min[3]; uint32_t start; float max[3]; uint32_t end; } nb_bvh_node;
+
+/* Vertex, src/pathtracing/scene.rs:13-21 (48 B). */
+typedef struct { float position[3]; float u; float normal[3]; float v; float tangent[4]; } nb_vertex;
+
+/* Instance, src/pathtracing/scene.rs:220-230 (160 B); matrices column-major. */
+typedef struct {
+    float world_to_local[16];
+    float local_to_world[16];
+    float color[4];
+    float roughness, metallic, emissive;
+    uint32_t node;
+} nb_instance;
+
+/* Primitive, src/pathtracing/scene.rs:59-67 (index_range as [index_start, index_end)). */
+typedef struct {
+    float local_to_world[16];
+    float color[4];
+    float roughness, metallic, emissive;
+    uint32_t index_start, index_end;
+} nb_primitive;
+
+/* CameraBuffer, src/common/camera.rs:7-12 (128 B). */
+typedef struct { float world_to_clip[16]; float clip_to_world[16]; } nb_camera;
+
+/* Globals push constants, src/pathtracing/pathtracer.rs:26-33 (16 B); sample is 1-based. */
+typedef struct { uint32_t sample; float weight; uint32_t bounces; float contribution_factor; } nb_globals;
+
+/* Per-ray counters (raytracing_sw.wgsl:99-101 n_aabb / n_tri, plus rays = intersect_scene
+ * calls, instance entries, finished samples and rays that hit), summed since the last reset. */
+typedef struct { uint64_t rays, n_aabb, n_tri, n_inst, samples, hits; } nb_counters;
+
+typedef struct nb_context nb_context;
+typedef struct nb_scene nb_scene;
+typedef struct nb_target nb_target;
+typedef struct nb_envmap nb_envmap;
+
+/* ---- errors / library ------------------------------------------------------------------ */
+const char *nb_last_error(void);
+/* Library version as MAJOR*10000 + MINOR*100 + PATCH. */
+uint32_t nb_version(void);
+
+/* ---- context: replaces WGPUContext (src/common/wgpu_context.rs:13-79) ------------------- */
+nb_status nb_context_create(int device, nb_context **out);
+nb_status nb_context_destroy(nb_context *ctx);
+nb_status nb_context_synchronize(nb_context *ctx);
+
+/* ---- host-side scene build ------------------------------------------------------------- */
+/* SceneBuffers::from_scene (src/pathtracing/scene.rs:280-307) without the upload: builds one
+ * BLAS per primitive into one shared node array (bvh.rs:162-190), the instance bounds
+ * (scene.rs:239-260), the TLAS (scene.rs:302) and writes the triangle permutation back into
+ * `indices` (scene.rs:305).  Bit-exact with the reference builder.
+ * flags: NB_BUILD_SHARE_BLAS reuses the first BLAS built over an index range for every later
+ * primitive over the same range instead of rebuilding it (the reference rebuilds).
+ * Capacities: see nb_build_scene_capacity. */
+#define NB_BUILD_SHARE_BLAS 1u
+nb_status nb_build_scene_capacity(uint32_t n_indices, const nb_primitive *prims, uint32_t n_prims,
+                                  uint32_t flags, uint32_t *blas_cap, uint32_t *tlas_cap);
+nb_status nb_build_scene(const nb_vertex *vertices, uint32_t n_vertices,
+                         uint32_t *indices, uint32_t n_indices,
+                         const nb_primitive *prims, uint32_t n_prims, uint32_t flags,
+                         nb_bvh_node *blas, uint32_t blas_cap, uint32_t *n_blas,
+                         nb_bvh_node *tlas, uint32_t tlas_cap, uint32_t *n_tlas,
+                         nb_instance *instances);
+
+/* ---- device scene: group-1 bindings blas, tlas, instances, vertices, indices ------------
+ * (src/pathtracing/raytracing_sw.wgsl:33-37, uploaded at scene.rs:309-341).  The arrays are
+ * validated (indices in range, children after parents, depth <= 48) and re-laid out for the
+ * GPU; the caller's arrays are not kept. */
+nb_status nb_scene_create(nb_context *ctx,
+                          const nb_bvh_node *blas, uint32_t n_blas,
+                          const nb_bvh_node *tlas, uint32_t n_tlas,
+                          const nb_instance *instances, uint32_t n_instances,
+                          const nb_vertex *vertices, uint32_t n_vertices,
+                          const uint32_t *indices, uint32_t n_indices,
+                          nb_scene **out);
+nb_status nb_scene_destroy(nb_scene *scene);
+/* Device bytes held by the scene. */
+nb_status nb_scene_device_bytes(const nb_scene *scene, uint64_t *bytes);
+
+/* ---- Sobol-Burley table (pathtracer.rs:57-71, binding pathtracing.wgsl:13) -------------
+ * nb_sobol_generate fills rows*dims float4 entries, entry [i*dims + set] = Owen-scrambled
+ * Sobol point i of dimension set `set` (four dimensions), Burley 2020.
+ * nb_sobol_set uploads a table (generated here or supplied by the caller, e.g. the exact
+ * table a reference run used); `stride` is the number of float4 entries per sample row the
+ * kernel steps by: 17 reproduces the reference (pathtracing.wgsl:2-4) whatever `bounces`. */
+nb_status nb_sobol_generate(uint32_t rows, uint32_t dims, uint32_t seed, float *out);
+nb_status nb_sobol_set(nb_context *ctx, const float *table, uint32_t rows, uint32_t dims, uint32_t stride);
+
+/* ---- environment map (EnvMap::load, src/pathtracing/envmap.rs:12; cube view + linear
+ * sampler, src/common/texture.rs:126-141) ----------------------------------------------
+ * texels: 6 faces in Vulkan layer order +X,-X,+Y,-Y,+Z,-Z, each face_size^2 RGBA32F, row-major. */
+nb_status nb_envmap_create_cube(nb_context *ctx, const float *texels, uint32_t face_size, nb_envmap **out);
+nb_status nb_envmap_destroy(nb_envmap *env);
+
+/* ---- camera (CameraController::calc_camera_data, src/common/camera.rs:121-142) ---------- */
+nb_status nb_camera_look_at(const float eye[3], const float target[3], const float up[3],
+                            float fov_y, float aspect, float z_near, nb_camera *out);
+
+/* ---- render target (Pathtracer::create_output_texture, pathtracer.rs:192-202) ----------
+ * RGBA32F, width and height multiples of 8 (pathtracer.rs:194). */
+nb_status nb_target_create(nb_context *ctx, uint32_t width, uint32_t height, nb_target **out);
+nb_status nb_target_destroy(nb_target *target);
+nb_status nb_target_read(nb_target *target, float *rgba);         /* waits for pending work */
+nb_status nb_target_write(nb_target *target, const float *rgba);
+nb_status nb_target_clear(nb_target *target);
+
+/* ---- rendering (Pathtracer::dispatch, pathtracer.rs:225-239 -> pathtracing.wgsl:main) ----
+ * nb_render: one sample per pixel of frame globals->sample with weight globals->weight,
+ * color = mix(previous, sample, weight) (pathtracing.wgsl:247-260).
+ * nb_render_frames: frames first->sample .. first->sample + n_frames - 1 in one launch, each
+ * with weight 1/sample; the image equals n_frames consecutive nb_render calls bit for bit. */
+nb_status nb_render(nb_context *ctx, const nb_scene *scene, nb_target *target, const nb_camera *camera,
+                    const nb_globals *globals, const nb_envmap *env);
+nb_status nb_render_frames(nb_context *ctx, const nb_scene *scene, nb_target *target, const nb_camera *camera,
+                           const nb_globals *first, uint32_t n_frames, const nb_envmap *env);
+
+/* Device time of the last render call in milliseconds (HIP events on the context stream). */
+nb_status nb_last_render_ms(nb_context *ctx, float *ms);
+/* HIP events around a region of work on the context stream: start records the first event,
+ * stop records the second, waits for it and returns the elapsed milliseconds. */
+nb_status nb_stream_timer_start(nb_context *ctx);
+nb_status nb_stream_timer_stop(nb_context *ctx, float *ms);
+
+/* ---- counters / debug ------------------------------------------------------------------- */
+nb_status nb_counters_get(nb_context *ctx, nb_counters *out);
+nb_status nb_counters_reset(nb_context *ctx);
+/* The closest hit of every primary ray of frame `sample` (the commented-out debug view of
+ * pathtracing.wgsl:262 over intersect_TLAS, raytracing_sw.wgsl:176): out[5*p..] =
+ * {instance, i0, i1, i2, float bits of dist}, instance = 0xFFFFFFFF on a miss. */
+nb_status nb_primary_hits(nb_context *ctx, const nb_scene *scene, const nb_camera *camera,
+                          uint32_t width, uint32_t height, uint32_t sample, uint32_t *out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,10 @@
+"""nbounce_amd: MI355X (gfx950) path-tracing kernel behind the reference's `pathtracing` surface.
+
+The compute path is libnbounce.so (HIP kernels + C ABI, include/nbounce.h); this package is its
+Python host mirror.  Importing it requires the built library.
+"""
+from .pathtracer import (CameraController, Context, EnvMap, Globals, NbError, Pathtracer, Scene, SceneBuffers,
+                         build_scene, sobol_table)
+
+__all__ = ["CameraController", "Context", "EnvMap", "Globals", "NbError", "Pathtracer", "Scene", "SceneBuffers",
+           "build_scene", "sobol_table"]

@@ -1,0 +1,95 @@
+"""ctypes binding of libnbounce.so (include/nbounce.h).
+
+The library is the only compute path: importing this module without the built library raises,
+there is no Python or CPU fallback.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+from . import layout
+
+LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libnbounce.so")
+
+_c = ctypes
+_p = _c.c_void_p
+_u32 = _c.c_uint32
+_f32 = _c.c_float
+_ptr = _c.POINTER
+
+# name -> argtypes (every entry point returns nb_status, an int, except where noted)
+SIGNATURES = {
+    "nb_context_create": [_c.c_int, _c.POINTER(_p)],
+    "nb_context_destroy": [_p],
+    "nb_context_synchronize": [_p],
+    "nb_build_scene_capacity": [_u32, _p, _u32, _u32, _ptr(_u32), _ptr(_u32)],
+    "nb_build_scene": [_p, _u32, _p, _u32, _p, _u32, _u32, _p, _u32, _ptr(_u32), _p, _u32, _ptr(_u32), _p],
+    "nb_scene_create": [_p, _p, _u32, _p, _u32, _p, _u32, _p, _u32, _p, _u32, _c.POINTER(_p)],
+    "nb_scene_destroy": [_p],
+    "nb_scene_device_bytes": [_p, _ptr(_c.c_uint64)],
+    "nb_sobol_generate": [_u32, _u32, _u32, _p],
+    "nb_sobol_set": [_p, _p, _u32, _u32, _u32],
+    "nb_envmap_create_cube": [_p, _p, _u32, _c.POINTER(_p)],
+    "nb_envmap_destroy": [_p],
+    "nb_camera_look_at": [_p, _p, _p, _f32, _f32, _f32, _p],
+    "nb_target_create": [_p, _u32, _u32, _c.POINTER(_p)],
+    "nb_target_destroy": [_p],
+    "nb_target_read": [_p, _p],
+    "nb_target_write": [_p, _p],
+    "nb_target_clear": [_p],
+    "nb_render": [_p, _p, _p, _p, _p, _p],
+    "nb_render_frames": [_p, _p, _p, _p, _p, _u32, _p],
+    "nb_last_render_ms": [_p, _ptr(_f32)],
+    "nb_stream_timer_start": [_p],
+    "nb_stream_timer_stop": [_p, _ptr(_f32)],
+    "nb_counters_get": [_p, _p],
+    "nb_counters_reset": [_p],
+    "nb_primary_hits": [_p, _p, _p, _u32, _u32, _u32, _p],
+}
+
+
+class NbError(RuntimeError):
+    """A call into libnbounce returned a nonzero nb_status."""
+
+    def __init__(self, status, message):
+        super().__init__(f"nb_status {status}: {message}")
+        self.status = status
+
+
+def _load():
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"{LIB_PATH} is not built; run `python nbounce_amd/build.py` (hipcc, gfx950)")
+    lib = ctypes.CDLL(LIB_PATH)
+    for name, args in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.argtypes = args
+        fn.restype = _c.c_int
+    lib.nb_last_error.argtypes = []
+    lib.nb_last_error.restype = _c.c_char_p
+    lib.nb_version.argtypes = []
+    lib.nb_version.restype = _u32
+    return lib
+
+
+lib = _load()
+
+
+def check(status):
+    if status != 0:
+        raise NbError(status, lib.nb_last_error().decode(errors="replace"))
+
+
+def ptr(a):
+    """Address of a C-contiguous numpy array (or None for a null pointer)."""
+    if a is None:
+        return None
+    assert a.flags["C_CONTIGUOUS"], "arrays crossing the ABI must be C-contiguous"
+    return a.ctypes.data
+
+
+def struct_bytes(dtype, **fields):
+    a = np.zeros(1, dtype=dtype)
+    for k, v in fields.items():
+        a[k] = v
+    return a

@@ -1,0 +1,694 @@
+// C ABI of libnbounce: contexts, device scenes, Sobol table, environment cube, render targets
+// and render calls (declared in include/nbounce.h).
+#include "nbounce.h"
+#include "nb_internal.h"
+
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+namespace nb {
+
+static thread_local std::string g_error;
+
+nb_status set_error(nb_status st, const char *fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    std::vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    g_error = buf;
+    return st;
+}
+
+#define NB_HIP(call)                                                                                   \
+    do {                                                                                               \
+        hipError_t e_ = (call);                                                                        \
+        if (e_ != hipSuccess)                                                                          \
+            return set_error(e_ == hipErrorOutOfMemory ? NB_ERR_OUT_OF_MEMORY : NB_ERR_HIP, "%s: %s", \
+                             #call, hipGetErrorString(e_));                                            \
+    } while (0)
+
+}  // namespace nb
+
+using namespace nb;
+
+struct nb_context {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev_start = nullptr, ev_stop = nullptr;
+    hipEvent_t ev_region_start = nullptr, ev_region_stop = nullptr;
+    bool timed = false, region_started = false;
+    float4 *sobol = nullptr;
+    uint32_t sobol_entries = 0, sobol_stride = 17;
+    unsigned long long *counters = nullptr;
+};
+
+struct nb_scene {
+    nb_context *ctx = nullptr;
+    void *block = nullptr;
+    uint64_t bytes = 0;
+    DevScene dev{};
+};
+
+struct nb_target {
+    nb_context *ctx = nullptr;
+    float4 *pixels = nullptr;
+    uint32_t width = 0, height = 0;
+};
+
+struct nb_envmap {
+    nb_context *ctx = nullptr;
+    float4 *texels = nullptr;
+    uint32_t n = 0;
+};
+
+namespace {
+
+nb_status use_device(const nb_context *ctx) {
+    NB_HIP(hipSetDevice(ctx->device));
+    return NB_OK;
+}
+
+#define NB_TRY(expr)                   \
+    do {                               \
+        nb_status s_ = (expr);         \
+        if (s_ != NB_OK) return s_;    \
+    } while (0)
+
+// ---- scene re-layout ------------------------------------------------------------------------
+
+struct Relayout {
+    std::vector<float4> pairs, tris, inst, ishade, vshade;
+    std::vector<uint32_t> tri_leaf, inst_leaf;
+    uint32_t tlas_root = kNoRecord;
+};
+
+// Validates one tree (children after parents, leaves inside [0, n_prims), depth <= 32).
+nb_status check_tree(const nb_bvh_node *nodes, uint32_t n, uint32_t n_prims, const char *name) {
+    std::vector<uint8_t> depth(n, 0);
+    for (uint32_t i = 0; i < n; ++i) {
+        const nb_bvh_node &d = nodes[i];
+        if (d.end > 0) {
+            if (d.start >= d.end || d.end > n_prims)
+                return set_error(NB_ERR_BAD_LAYOUT, "%s node %u: leaf range [%u, %u) outside [0, %u)", name, i, d.start, d.end, n_prims);
+            if (d.start > kLeafStartMask)
+                return set_error(NB_ERR_UNSUPPORTED, "%s node %u: leaf start %u exceeds 2^26", name, i, d.start);
+        } else {
+            if (d.start <= i || d.start + 1 >= n)
+                return set_error(NB_ERR_BAD_LAYOUT, "%s node %u: children %u, %u not after it", name, i, d.start, d.start + 1);
+            uint8_t cd = static_cast<uint8_t>(depth[i] + 1);
+            if (cd > kMaxTreeDepth)
+                return set_error(NB_ERR_BAD_LAYOUT, "%s deeper than %u levels", name, kMaxTreeDepth);
+            if (cd > depth[d.start]) depth[d.start] = cd;
+            if (cd > depth[d.start + 1]) depth[d.start + 1] = cd;
+        }
+    }
+    return NB_OK;
+}
+
+nb_status child_ref(const nb_bvh_node &n, const std::vector<uint32_t> &pid, uint32_t idx, std::vector<uint32_t> &side,
+                    uint32_t *out) {
+    if (n.end == 0) { *out = pid[idx]; return NB_OK; }
+    uint32_t count = n.end - n.start;
+    uint32_t field = count <= kLeafMaxInline ? count : 0u;
+    if (field == 0) {
+        if (side[n.start] != 0 && side[n.start] != count)
+            return set_error(NB_ERR_UNSUPPORTED, "two leaves of more than %u primitives start at %u with different counts",
+                             kLeafMaxInline, n.start);
+        side[n.start] = count;
+    }
+    *out = kLeafBit | (field << kLeafStartBits) | n.start;
+    return NB_OK;
+}
+
+void push_pair(std::vector<float4> &pairs, const nb_bvh_node &l, uint32_t lref, const nb_bvh_node *r, uint32_t rref,
+               uint32_t is_root) {
+    const float nan = std::nanf("");
+    float rmin[3] = {nan, nan, nan}, rmax[3] = {nan, nan, nan};
+    if (r) { std::memcpy(rmin, r->min, 12); std::memcpy(rmax, r->max, 12); }
+    pairs.push_back(make_float4(l.min[0], l.min[1], l.min[2], l.max[0]));
+    pairs.push_back(make_float4(l.max[1], l.max[2], rmin[0], rmin[1]));
+    pairs.push_back(make_float4(rmin[2], rmax[0], rmax[1], rmax[2]));
+    float4 q3;
+    uint32_t w[4] = {lref, rref, is_root, 0u};
+    std::memcpy(&q3, w, 16);
+    pairs.push_back(q3);
+}
+
+nb_status relayout(const nb_bvh_node *blas, uint32_t n_blas, const nb_bvh_node *tlas, uint32_t n_tlas,
+                   const nb_instance *inst, uint32_t n_inst, const nb_vertex *vert, uint32_t n_vert,
+                   const uint32_t *idx, uint32_t n_idx, Relayout &R) {
+    const uint32_t n_tris = n_idx / 3;
+    for (uint32_t i = 0; i < n_tris * 3; ++i)
+        if (idx[i] >= n_vert) return set_error(NB_ERR_BAD_LAYOUT, "index %u = %u out of range (%u vertices)", i, idx[i], n_vert);
+    NB_TRY(check_tree(blas, n_blas, n_tris, "BLAS"));
+    NB_TRY(check_tree(tlas, n_tlas, n_inst, "TLAS"));
+    if (n_tlas > 0 && n_inst == 0) return set_error(NB_ERR_BAD_LAYOUT, "TLAS without instances");
+    if (n_inst > kLeafStartMask) return set_error(NB_ERR_UNSUPPORTED, "more than 2^26 instances");
+    for (uint32_t j = 0; j < n_inst; ++j)
+        if (inst[j].node >= n_blas) return set_error(NB_ERR_BAD_LAYOUT, "instance %u: BLAS root %u out of range", j, inst[j].node);
+
+    // record ids: BLAS inner nodes, BLAS root records, TLAS inner nodes, TLAS root record
+    std::vector<uint32_t> pid_b(n_blas, kNoRecord), pid_t(n_tlas, kNoRecord);
+    uint32_t next = 0;
+    for (uint32_t i = 0; i < n_blas; ++i) if (blas[i].end == 0) pid_b[i] = next++;
+    std::map<uint32_t, uint32_t> root_rec;
+    for (uint32_t j = 0; j < n_inst; ++j) root_rec.emplace(inst[j].node, 0u);
+    for (auto &kv : root_rec) kv.second = next++;
+    for (uint32_t i = 0; i < n_tlas; ++i) if (tlas[i].end == 0) pid_t[i] = next++;
+    if (n_tlas > 0) R.tlas_root = next++;
+
+    R.tri_leaf.assign(n_tris ? n_tris : 1, 0u);
+    R.inst_leaf.assign(n_inst ? n_inst : 1, 0u);
+    R.pairs.reserve(4 * static_cast<size_t>(next));
+    for (uint32_t i = 0; i < n_blas; ++i) {
+        if (blas[i].end != 0) continue;
+        uint32_t l = blas[i].start, r = l + 1, lr, rr;
+        NB_TRY(child_ref(blas[l], pid_b, l, R.tri_leaf, &lr));
+        NB_TRY(child_ref(blas[r], pid_b, r, R.tri_leaf, &rr));
+        push_pair(R.pairs, blas[l], lr, &blas[r], rr, 0u);
+    }
+    for (auto &kv : root_rec) {
+        uint32_t ref;
+        NB_TRY(child_ref(blas[kv.first], pid_b, kv.first, R.tri_leaf, &ref));
+        push_pair(R.pairs, blas[kv.first], ref, nullptr, 0u, 1u);
+    }
+    for (uint32_t i = 0; i < n_tlas; ++i) {
+        if (tlas[i].end != 0) continue;
+        uint32_t l = tlas[i].start, r = l + 1, lr, rr;
+        NB_TRY(child_ref(tlas[l], pid_t, l, R.inst_leaf, &lr));
+        NB_TRY(child_ref(tlas[r], pid_t, r, R.inst_leaf, &rr));
+        push_pair(R.pairs, tlas[l], lr, &tlas[r], rr, 0u);
+    }
+    if (n_tlas > 0) {
+        uint32_t ref;
+        NB_TRY(child_ref(tlas[0], pid_t, 0, R.inst_leaf, &ref));
+        push_pair(R.pairs, tlas[0], ref, nullptr, 0u, 1u);
+    }
+
+    R.tris.resize(3 * static_cast<size_t>(n_tris));
+    for (uint32_t t = 0; t < n_tris; ++t) {
+        const uint32_t i0 = idx[3 * t], i1 = idx[3 * t + 1], i2 = idx[3 * t + 2];
+        const float *p0 = vert[i0].position, *p1 = vert[i1].position, *p2 = vert[i2].position;
+        float b0, b1, b2;
+        std::memcpy(&b0, &i0, 4); std::memcpy(&b1, &i1, 4); std::memcpy(&b2, &i2, 4);
+        R.tris[3 * t + 0] = make_float4(p0[0], p0[1], p0[2], b0);
+        R.tris[3 * t + 1] = make_float4(p1[0] - p0[0], p1[1] - p0[1], p1[2] - p0[2], b1);
+        R.tris[3 * t + 2] = make_float4(p2[0] - p0[0], p2[1] - p0[1], p2[2] - p0[2], b2);
+    }
+    R.inst.resize(4 * static_cast<size_t>(n_inst));
+    R.ishade.resize(6 * static_cast<size_t>(n_inst));
+    for (uint32_t j = 0; j < n_inst; ++j) {
+        const float *w = inst[j].world_to_local, *l = inst[j].local_to_world;
+        for (int r = 0; r < 3; ++r) R.inst[4 * j + r] = make_float4(w[0 + r], w[4 + r], w[8 + r], w[12 + r]);
+        float root_bits;
+        uint32_t rid = root_rec[inst[j].node];
+        std::memcpy(&root_bits, &rid, 4);
+        R.inst[4 * j + 3] = make_float4(root_bits, 0.0f, 0.0f, 0.0f);
+        R.ishade[6 * j + 0] = make_float4(w[0], w[1], w[2], inst[j].roughness);
+        R.ishade[6 * j + 1] = make_float4(w[4], w[5], w[6], inst[j].metallic);
+        R.ishade[6 * j + 2] = make_float4(w[8], w[9], w[10], inst[j].emissive);
+        R.ishade[6 * j + 3] = make_float4(l[0], l[1], l[2], inst[j].color[0]);
+        R.ishade[6 * j + 4] = make_float4(l[4], l[5], l[6], inst[j].color[1]);
+        R.ishade[6 * j + 5] = make_float4(l[8], l[9], l[10], inst[j].color[2]);
+    }
+    R.vshade.resize(2 * static_cast<size_t>(n_vert));
+    for (uint32_t v = 0; v < n_vert; ++v) {
+        const nb_vertex &x = vert[v];
+        R.vshade[2 * v] = make_float4(x.normal[0], x.normal[1], x.normal[2], 0.0f);
+        R.vshade[2 * v + 1] = make_float4(x.tangent[0], x.tangent[1], x.tangent[2], x.tangent[3]);
+    }
+    return NB_OK;
+}
+
+// ---- cube borders ---------------------------------------------------------------------------
+// Vulkan cube faces: major axis (index, sign), then the (axis, sign) giving sc and tc.
+struct FaceAxes { int ma; float ms; int sa; float ss; int ta; float ts; };
+constexpr FaceAxes kFaces[6] = {
+    {0, 1, 2, -1, 1, -1}, {0, -1, 2, 1, 1, -1}, {1, 1, 0, 1, 2, 1},
+    {1, -1, 0, 1, 2, -1}, {2, 1, 0, 1, 1, -1}, {2, -1, 0, -1, 1, -1},
+};
+
+// The texel across the edge of face f for (i, j) with exactly one coordinate one step outside.
+void across_edge(int f, int i, int j, int n, int &g, int &gi, int &gj) {
+    const FaceAxes &F = kFaces[f];
+    const bool s_out = i < 0 || i >= n;
+    const float side = s_out ? (i < 0 ? -1.0f : 1.0f) : (j < 0 ? -1.0f : 1.0f);
+    const int out_axis = s_out ? F.sa : F.ta;
+    const float out_sign = side * (s_out ? F.ss : F.ts);     // direction component on the out axis
+    const int along_axis = s_out ? F.ta : F.sa;
+    const float along_sign = s_out ? F.ts : F.ss;
+    const int along = s_out ? j : i;
+    g = out_axis * 2 + (out_sign > 0.0f ? 0 : 1);
+    const FaceAxes &G = kFaces[g];
+    // On g, the axis that was f's major axis sits at the edge (value F.ms); the other keeps `along`.
+    if (G.sa == F.ma) {
+        gi = G.ss * F.ms > 0.0f ? n - 1 : 0;
+        gj = G.ts * along_sign > 0.0f ? along : n - 1 - along;
+    } else {
+        gj = G.ts * F.ms > 0.0f ? n - 1 : 0;
+        gi = G.ss * along_sign > 0.0f ? along : n - 1 - along;
+    }
+    (void)along_axis;
+}
+
+float4 face_texel(const float *tex, int n, int f, int i, int j) {
+    const float *t = tex + 4 * ((static_cast<size_t>(f) * n + j) * n + i);
+    return make_float4(t[0], t[1], t[2], t[3]);
+}
+
+// Padded faces: (n+2)^2 texels each; border texels hold the seamless neighbours, corners the
+// average of the face's corner texel and its two neighbours across the edges.
+std::vector<float4> pad_cube(const float *tex, int n) {
+    const int pn = n + 2;
+    std::vector<float4> out(6 * static_cast<size_t>(pn) * pn);
+    for (int f = 0; f < 6; ++f)
+        for (int pj = 0; pj < pn; ++pj)
+            for (int pi = 0; pi < pn; ++pi) {
+                const int i = pi - 1, j = pj - 1;
+                const bool oi = i < 0 || i >= n, oj = j < 0 || j >= n;
+                float4 v;
+                if (!oi && !oj) {
+                    v = face_texel(tex, n, f, i, j);
+                } else if (oi && oj) {
+                    const int ci = i < 0 ? 0 : n - 1, cj = j < 0 ? 0 : n - 1;
+                    int g1, i1, j1, g2, i2, j2;
+                    across_edge(f, i, cj, n, g1, i1, j1);
+                    across_edge(f, ci, j, n, g2, i2, j2);
+                    float4 a = face_texel(tex, n, f, ci, cj), b = face_texel(tex, n, g1, i1, j1),
+                           c = face_texel(tex, n, g2, i2, j2);
+                    v = make_float4(((a.x + b.x) + c.x) / 3.0f, ((a.y + b.y) + c.y) / 3.0f, ((a.z + b.z) + c.z) / 3.0f,
+                                    ((a.w + b.w) + c.w) / 3.0f);
+                } else {
+                    int g, gi, gj;
+                    across_edge(f, i, j, n, g, gi, gj);
+                    v = face_texel(tex, n, g, gi, gj);
+                }
+                out[(static_cast<size_t>(f) * pn + pj) * pn + pi] = v;
+            }
+    return out;
+}
+
+nb_status check_render_args(nb_context *ctx, const nb_scene *scene, nb_target *target, const nb_camera *camera,
+                            const nb_globals *g, const nb_envmap *env) {
+    if (!ctx || !scene || !target || !camera || !g || !env)
+        return set_error(NB_ERR_INVALID_ARGUMENT, "null argument");
+    if (scene->ctx != ctx || target->ctx != ctx || env->ctx != ctx)
+        return set_error(NB_ERR_INVALID_ARGUMENT, "scene, target and environment must belong to the context");
+    if (!ctx->sobol) return set_error(NB_ERR_INVALID_ARGUMENT, "no Sobol table set (nb_sobol_set)");
+    return NB_OK;
+}
+
+RenderParams make_params(nb_context *ctx, const nb_scene *scene, nb_target *t, const nb_camera *cam,
+                         const nb_globals *g, const nb_envmap *env) {
+    RenderParams p{};
+    p.scene = scene->dev;
+    p.env = DevEnv{env ? env->texels : nullptr, env ? env->n : 0u};
+    p.target = t->pixels;
+    p.width = t->width;
+    p.height = t->height;
+    std::memcpy(p.c2w, cam->clip_to_world, sizeof(p.c2w));
+    p.sobol = ctx->sobol;
+    p.sobol_entries = ctx->sobol_entries;
+    p.sobol_stride = ctx->sobol_stride;
+    p.first_sample = g->sample;
+    p.n_frames = 1;
+    p.first_weight = g->weight;
+    p.weights_from_sample = 0;
+    p.bounces = g->bounces;
+    p.contribution_factor = g->contribution_factor;
+    p.counters = ctx->counters;
+    p.tiles_x = t->width / 8;
+    p.n_tiles = (t->width / 8) * (t->height / 8);
+    p.tiles_per_xcd = (p.n_tiles + 7) / 8;
+    p.hits_out = nullptr;
+    return p;
+}
+
+nb_status timed_launch(nb_context *ctx, const RenderParams &p, nb_status (*launch)(const RenderParams &, void *)) {
+    NB_HIP(hipEventRecord(ctx->ev_start, ctx->stream));
+    NB_TRY(launch(p, ctx->stream));
+    NB_HIP(hipEventRecord(ctx->ev_stop, ctx->stream));
+    ctx->timed = true;
+    return NB_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *nb_last_error(void) { return g_error.c_str(); }
+
+uint32_t nb_version(void) { return 100; }
+
+nb_status nb_context_create(int device, nb_context **out) {
+    if (!out) return set_error(NB_ERR_INVALID_ARGUMENT, "null output");
+    *out = nullptr;
+    int n = 0;
+    NB_HIP(hipGetDeviceCount(&n));
+    if (device < 0 || device >= n) return set_error(NB_ERR_INVALID_ARGUMENT, "device %d not present (%d devices)", device, n);
+    NB_HIP(hipSetDevice(device));
+    nb_context *c = new nb_context();
+    c->device = device;
+    hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreate(&c->ev_start);
+    if (e == hipSuccess) e = hipEventCreate(&c->ev_stop);
+    if (e == hipSuccess) e = hipEventCreate(&c->ev_region_start);
+    if (e == hipSuccess) e = hipEventCreate(&c->ev_region_stop);
+    if (e == hipSuccess) e = hipMalloc(&c->counters, 8 * sizeof(unsigned long long));
+    if (e == hipSuccess) e = hipMemset(c->counters, 0, 8 * sizeof(unsigned long long));
+    if (e != hipSuccess) {
+        nb_context_destroy(c);
+        return set_error(NB_ERR_HIP, "context creation: %s", hipGetErrorString(e));
+    }
+    *out = c;
+    return NB_OK;
+}
+
+nb_status nb_context_destroy(nb_context *ctx) {
+    if (!ctx) return NB_OK;
+    hipSetDevice(ctx->device);
+    if (ctx->stream) hipStreamSynchronize(ctx->stream);
+    hipFree(ctx->sobol);
+    hipFree(ctx->counters);
+    if (ctx->ev_start) hipEventDestroy(ctx->ev_start);
+    if (ctx->ev_stop) hipEventDestroy(ctx->ev_stop);
+    if (ctx->ev_region_start) hipEventDestroy(ctx->ev_region_start);
+    if (ctx->ev_region_stop) hipEventDestroy(ctx->ev_region_stop);
+    if (ctx->stream) hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return NB_OK;
+}
+
+nb_status nb_context_synchronize(nb_context *ctx) {
+    if (!ctx) return set_error(NB_ERR_INVALID_ARGUMENT, "null context");
+    NB_TRY(use_device(ctx));
+    NB_HIP(hipStreamSynchronize(ctx->stream));
+    return NB_OK;
+}
+
+nb_status nb_build_scene_capacity(uint32_t n_indices, const nb_primitive *prims, uint32_t n_prims, uint32_t flags,
+                                  uint32_t *blas_cap, uint32_t *tlas_cap) {
+    if (!prims || !blas_cap || !tlas_cap) return set_error(NB_ERR_INVALID_ARGUMENT, "null argument");
+    return build_scene_capacity(n_indices, prims, n_prims, flags, blas_cap, tlas_cap);
+}
+
+nb_status nb_build_scene(const nb_vertex *vertices, uint32_t n_vertices, uint32_t *indices, uint32_t n_indices,
+                         const nb_primitive *prims, uint32_t n_prims, uint32_t flags, nb_bvh_node *blas,
+                         uint32_t blas_cap, uint32_t *n_blas, nb_bvh_node *tlas, uint32_t tlas_cap, uint32_t *n_tlas,
+                         nb_instance *instances) {
+    if (!vertices || !indices || !prims || !blas || !n_blas || !tlas || !n_tlas || !instances)
+        return set_error(NB_ERR_INVALID_ARGUMENT, "null argument");
+    try {
+        return build_scene(vertices, n_vertices, indices, n_indices, prims, n_prims, flags, blas, blas_cap, n_blas,
+                           tlas, tlas_cap, n_tlas, instances);
+    } catch (const std::bad_alloc &) {
+        return set_error(NB_ERR_OUT_OF_MEMORY, "host allocation failed in scene build");
+    }
+}
+
+nb_status nb_scene_create(nb_context *ctx, const nb_bvh_node *blas, uint32_t n_blas, const nb_bvh_node *tlas,
+                          uint32_t n_tlas, const nb_instance *instances, uint32_t n_instances, const nb_vertex *vertices,
+                          uint32_t n_vertices, const uint32_t *indices, uint32_t n_indices, nb_scene **out) {
+    if (!ctx || !out) return set_error(NB_ERR_INVALID_ARGUMENT, "null argument");
+    *out = nullptr;
+    if ((n_blas && !blas) || (n_tlas && !tlas) || (n_instances && !instances) || (n_vertices && !vertices) ||
+        (n_indices && !indices))
+        return set_error(NB_ERR_INVALID_ARGUMENT, "null array with nonzero length");
+    NB_TRY(use_device(ctx));
+    Relayout R;
+    try {
+        NB_TRY(relayout(blas, n_blas, tlas, n_tlas, instances, n_instances, vertices, n_vertices, indices, n_indices, R));
+    } catch (const std::bad_alloc &) {
+        return set_error(NB_ERR_OUT_OF_MEMORY, "host allocation failed in scene upload");
+    }
+    auto bytes_of = [](size_t n) { return (n * 4 + 255) / 256 * 256; };
+    const size_t sizes[7] = {R.pairs.size() * 16, R.tris.size() * 16, R.inst.size() * 16, R.ishade.size() * 16,
+                             R.vshade.size() * 16, R.tri_leaf.size() * 4, R.inst_leaf.size() * 4};
+    size_t total = 0, off[7];
+    for (int k = 0; k < 7; ++k) { off[k] = total; total += bytes_of((sizes[k] + 3) / 4); }
+    nb_scene *s = new nb_scene();
+    s->ctx = ctx;
+    hipError_t e = hipMalloc(&s->block, total ? total : 256);
+    if (e != hipSuccess) {
+        delete s;
+        return set_error(e == hipErrorOutOfMemory ? NB_ERR_OUT_OF_MEMORY : NB_ERR_HIP, "scene allocation (%zu B): %s",
+                         total, hipGetErrorString(e));
+    }
+    s->bytes = total;
+    char *b = static_cast<char *>(s->block);
+    const void *src[7] = {R.pairs.data(), R.tris.data(), R.inst.data(), R.ishade.data(), R.vshade.data(),
+                          R.tri_leaf.data(), R.inst_leaf.data()};
+    for (int k = 0; k < 7; ++k) {
+        if (!sizes[k]) continue;
+        e = hipMemcpy(b + off[k], src[k], sizes[k], hipMemcpyHostToDevice);
+        if (e != hipSuccess) {
+            hipFree(s->block);
+            delete s;
+            return set_error(NB_ERR_HIP, "scene upload: %s", hipGetErrorString(e));
+        }
+    }
+    s->dev.pairs = reinterpret_cast<const float4 *>(b + off[0]);
+    s->dev.tris = reinterpret_cast<const float4 *>(b + off[1]);
+    s->dev.inst = reinterpret_cast<const float4 *>(b + off[2]);
+    s->dev.ishade = reinterpret_cast<const float4 *>(b + off[3]);
+    s->dev.vshade = reinterpret_cast<const float4 *>(b + off[4]);
+    s->dev.tri_leaf_count = reinterpret_cast<const uint32_t *>(b + off[5]);
+    s->dev.inst_leaf_count = reinterpret_cast<const uint32_t *>(b + off[6]);
+    s->dev.tlas_root = R.tlas_root;
+    s->dev.n_pairs = static_cast<uint32_t>(R.pairs.size() / 4);
+    *out = s;
+    return NB_OK;
+}
+
+nb_status nb_scene_destroy(nb_scene *scene) {
+    if (!scene) return NB_OK;
+    hipSetDevice(scene->ctx->device);
+    hipStreamSynchronize(scene->ctx->stream);
+    hipFree(scene->block);
+    delete scene;
+    return NB_OK;
+}
+
+nb_status nb_scene_device_bytes(const nb_scene *scene, uint64_t *bytes) {
+    if (!scene || !bytes) return set_error(NB_ERR_INVALID_ARGUMENT, "null argument");
+    *bytes = scene->bytes;
+    return NB_OK;
+}
+
+nb_status nb_sobol_generate(uint32_t rows, uint32_t dims, uint32_t seed, float *out) {
+    if (!out) return set_error(NB_ERR_INVALID_ARGUMENT, "null output");
+    return sobol_generate(rows, dims, seed, out);
+}
+
+nb_status nb_sobol_set(nb_context *ctx, const float *table, uint32_t rows, uint32_t dims, uint32_t stride) {
+    if (!ctx || !table || rows == 0 || dims == 0 || stride == 0)
+        return set_error(NB_ERR_INVALID_ARGUMENT, "null table or zero size");
+    uint64_t entries = static_cast<uint64_t>(rows) * dims;
+    if (entries > 0x7FFFFFFFull) return set_error(NB_ERR_INVALID_ARGUMENT, "table too large");
+    NB_TRY(use_device(ctx));
+    NB_HIP(hipStreamSynchronize(ctx->stream));
+    float4 *d = nullptr;
+    NB_HIP(hipMalloc(&d, entries * 16));
+    hipError_t e = hipMemcpy(d, table, entries * 16, hipMemcpyHostToDevice);
+    if (e != hipSuccess) { hipFree(d); return set_error(NB_ERR_HIP, "table upload: %s", hipGetErrorString(e)); }
+    hipFree(ctx->sobol);
+    ctx->sobol = d;
+    ctx->sobol_entries = static_cast<uint32_t>(entries);
+    ctx->sobol_stride = stride;
+    return NB_OK;
+}
+
+nb_status nb_envmap_create_cube(nb_context *ctx, const float *texels, uint32_t face_size, nb_envmap **out) {
+    if (!ctx || !texels || !out || face_size == 0 || face_size > 16384)
+        return set_error(NB_ERR_INVALID_ARGUMENT, "bad environment arguments");
+    *out = nullptr;
+    NB_TRY(use_device(ctx));
+    std::vector<float4> padded = pad_cube(texels, static_cast<int>(face_size));
+    nb_envmap *m = new nb_envmap();
+    m->ctx = ctx;
+    m->n = face_size;
+    hipError_t e = hipMalloc(&m->texels, padded.size() * 16);
+    if (e == hipSuccess) e = hipMemcpy(m->texels, padded.data(), padded.size() * 16, hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        hipFree(m->texels);
+        delete m;
+        return set_error(NB_ERR_HIP, "environment upload: %s", hipGetErrorString(e));
+    }
+    *out = m;
+    return NB_OK;
+}
+
+nb_status nb_envmap_destroy(nb_envmap *env) {
+    if (!env) return NB_OK;
+    hipSetDevice(env->ctx->device);
+    hipStreamSynchronize(env->ctx->stream);
+    hipFree(env->texels);
+    delete env;
+    return NB_OK;
+}
+
+nb_status nb_camera_look_at(const float eye[3], const float target[3], const float up[3], float fov_y, float aspect,
+                            float z_near, nb_camera *out) {
+    if (!eye || !target || !up || !out) return set_error(NB_ERR_INVALID_ARGUMENT, "null argument");
+    camera_look_at(eye, target, up, fov_y, aspect, z_near, out);
+    return NB_OK;
+}
+
+nb_status nb_target_create(nb_context *ctx, uint32_t width, uint32_t height, nb_target **out) {
+    if (!ctx || !out) return set_error(NB_ERR_INVALID_ARGUMENT, "null argument");
+    *out = nullptr;
+    if (width == 0 || height == 0 || width % 8 || height % 8 || width > 32768 || height > 32768)
+        return set_error(NB_ERR_INVALID_ARGUMENT, "target %ux%u: sides must be nonzero multiples of 8", width, height);
+    NB_TRY(use_device(ctx));
+    nb_target *t = new nb_target();
+    t->ctx = ctx;
+    t->width = width;
+    t->height = height;
+    const size_t bytes = static_cast<size_t>(width) * height * 16;
+    hipError_t e = hipMalloc(&t->pixels, bytes);
+    if (e == hipSuccess) e = hipMemset(t->pixels, 0, bytes);
+    if (e != hipSuccess) {
+        hipFree(t->pixels);
+        delete t;
+        return set_error(e == hipErrorOutOfMemory ? NB_ERR_OUT_OF_MEMORY : NB_ERR_HIP, "target allocation: %s",
+                         hipGetErrorString(e));
+    }
+    *out = t;
+    return NB_OK;
+}
+
+nb_status nb_target_destroy(nb_target *t) {
+    if (!t) return NB_OK;
+    hipSetDevice(t->ctx->device);
+    hipStreamSynchronize(t->ctx->stream);
+    hipFree(t->pixels);
+    delete t;
+    return NB_OK;
+}
+
+nb_status nb_target_read(nb_target *t, float *rgba) {
+    if (!t || !rgba) return set_error(NB_ERR_INVALID_ARGUMENT, "null argument");
+    NB_TRY(use_device(t->ctx));
+    NB_HIP(hipStreamSynchronize(t->ctx->stream));
+    NB_HIP(hipMemcpy(rgba, t->pixels, static_cast<size_t>(t->width) * t->height * 16, hipMemcpyDeviceToHost));
+    return NB_OK;
+}
+
+nb_status nb_target_write(nb_target *t, const float *rgba) {
+    if (!t || !rgba) return set_error(NB_ERR_INVALID_ARGUMENT, "null argument");
+    NB_TRY(use_device(t->ctx));
+    NB_HIP(hipStreamSynchronize(t->ctx->stream));
+    NB_HIP(hipMemcpy(t->pixels, rgba, static_cast<size_t>(t->width) * t->height * 16, hipMemcpyHostToDevice));
+    return NB_OK;
+}
+
+nb_status nb_target_clear(nb_target *t) {
+    if (!t) return set_error(NB_ERR_INVALID_ARGUMENT, "null target");
+    NB_TRY(use_device(t->ctx));
+    NB_HIP(hipMemsetAsync(t->pixels, 0, static_cast<size_t>(t->width) * t->height * 16, t->ctx->stream));
+    return NB_OK;
+}
+
+nb_status nb_render(nb_context *ctx, const nb_scene *scene, nb_target *target, const nb_camera *camera,
+                    const nb_globals *globals, const nb_envmap *env) {
+    NB_TRY(check_render_args(ctx, scene, target, camera, globals, env));
+    NB_TRY(use_device(ctx));
+    RenderParams p = make_params(ctx, scene, target, camera, globals, env);
+    return timed_launch(ctx, p, launch_render);
+}
+
+nb_status nb_render_frames(nb_context *ctx, const nb_scene *scene, nb_target *target, const nb_camera *camera,
+                           const nb_globals *first, uint32_t n_frames, const nb_envmap *env) {
+    NB_TRY(check_render_args(ctx, scene, target, camera, first, env));
+    if (n_frames == 0) return NB_OK;
+    if (first->sample == 0 || first->sample > 0xFFFFFFFFu - n_frames)
+        return set_error(NB_ERR_INVALID_ARGUMENT, "frames must have 1-based sample numbers");
+    NB_TRY(use_device(ctx));
+    RenderParams p = make_params(ctx, scene, target, camera, first, env);
+    p.n_frames = n_frames;
+    p.weights_from_sample = 1;
+    return timed_launch(ctx, p, launch_render);
+}
+
+nb_status nb_last_render_ms(nb_context *ctx, float *ms) {
+    if (!ctx || !ms) return set_error(NB_ERR_INVALID_ARGUMENT, "null argument");
+    if (!ctx->timed) return set_error(NB_ERR_INVALID_ARGUMENT, "no render recorded");
+    NB_TRY(use_device(ctx));
+    NB_HIP(hipEventSynchronize(ctx->ev_stop));
+    NB_HIP(hipEventElapsedTime(ms, ctx->ev_start, ctx->ev_stop));
+    return NB_OK;
+}
+
+nb_status nb_stream_timer_start(nb_context *ctx) {
+    if (!ctx) return set_error(NB_ERR_INVALID_ARGUMENT, "null context");
+    NB_TRY(use_device(ctx));
+    NB_HIP(hipEventRecord(ctx->ev_region_start, ctx->stream));
+    ctx->region_started = true;
+    return NB_OK;
+}
+
+nb_status nb_stream_timer_stop(nb_context *ctx, float *ms) {
+    if (!ctx || !ms) return set_error(NB_ERR_INVALID_ARGUMENT, "null argument");
+    if (!ctx->region_started) return set_error(NB_ERR_INVALID_ARGUMENT, "timer not started");
+    NB_TRY(use_device(ctx));
+    NB_HIP(hipEventRecord(ctx->ev_region_stop, ctx->stream));
+    NB_HIP(hipEventSynchronize(ctx->ev_region_stop));
+    NB_HIP(hipEventElapsedTime(ms, ctx->ev_region_start, ctx->ev_region_stop));
+    ctx->region_started = false;
+    return NB_OK;
+}
+
+nb_status nb_counters_get(nb_context *ctx, nb_counters *out) {
+    if (!ctx || !out) return set_error(NB_ERR_INVALID_ARGUMENT, "null argument");
+    NB_TRY(use_device(ctx));
+    NB_HIP(hipStreamSynchronize(ctx->stream));
+    unsigned long long v[8];
+    NB_HIP(hipMemcpy(v, ctx->counters, sizeof(v), hipMemcpyDeviceToHost));
+    out->rays = v[0]; out->n_aabb = v[1]; out->n_tri = v[2]; out->n_inst = v[3]; out->samples = v[4]; out->hits = v[5];
+    return NB_OK;
+}
+
+nb_status nb_counters_reset(nb_context *ctx) {
+    if (!ctx) return set_error(NB_ERR_INVALID_ARGUMENT, "null context");
+    NB_TRY(use_device(ctx));
+    NB_HIP(hipMemsetAsync(ctx->counters, 0, 8 * sizeof(unsigned long long), ctx->stream));
+    return NB_OK;
+}
+
+nb_status nb_primary_hits(nb_context *ctx, const nb_scene *scene, const nb_camera *camera, uint32_t width,
+                          uint32_t height, uint32_t sample, uint32_t *out) {
+    if (!ctx || !scene || !camera || !out) return set_error(NB_ERR_INVALID_ARGUMENT, "null argument");
+    if (scene->ctx != ctx) return set_error(NB_ERR_INVALID_ARGUMENT, "scene belongs to another context");
+    if (!ctx->sobol) return set_error(NB_ERR_INVALID_ARGUMENT, "no Sobol table set (nb_sobol_set)");
+    if (width == 0 || height == 0 || width % 8 || height % 8)
+        return set_error(NB_ERR_INVALID_ARGUMENT, "sides must be nonzero multiples of 8");
+    NB_TRY(use_device(ctx));
+    const size_t bytes = static_cast<size_t>(width) * height * 5 * 4;
+    uint32_t *d = nullptr;
+    NB_HIP(hipMalloc(&d, bytes));
+    nb_target tmp;
+    tmp.ctx = ctx;
+    tmp.width = width;
+    tmp.height = height;
+    nb_globals g{sample, 0.0f, 0u, 0.0f};
+    RenderParams p = make_params(ctx, scene, &tmp, camera, &g, nullptr);
+    p.hits_out = d;
+    p.counters = nullptr;
+    nb_status st = launch_primary_hits(p, ctx->stream);
+    hipError_t e = hipStreamSynchronize(ctx->stream);
+    if (st == NB_OK && e == hipSuccess) e = hipMemcpy(out, d, bytes, hipMemcpyDeviceToHost);
+    hipFree(d);
+    if (st != NB_OK) return st;
+    if (e != hipSuccess) return set_error(NB_ERR_HIP, "primary hits: %s", hipGetErrorString(e));
+    return NB_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,89 @@
+// Internal declarations shared by the host files of libnbounce and the HIP kernels.
+#pragma once
+
+#include "nbounce.h"
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace nb {
+
+// Records the thread-local error message and returns `st`.
+nb_status set_error(nb_status st, const char *fmt, ...);
+
+nb_status build_scene_capacity(uint32_t n_indices, const nb_primitive *prims, uint32_t n_prims, uint32_t flags,
+                               uint32_t *blas_cap, uint32_t *tlas_cap);
+nb_status build_scene(const nb_vertex *vertices, uint32_t n_vertices, uint32_t *indices, uint32_t n_indices,
+                      const nb_primitive *prims, uint32_t n_prims, uint32_t flags,
+                      nb_bvh_node *blas, uint32_t blas_cap, uint32_t *n_blas,
+                      nb_bvh_node *tlas, uint32_t tlas_cap, uint32_t *n_tlas, nb_instance *instances);
+void camera_look_at(const float *eye, const float *target, const float *up, float fov_y, float aspect,
+                    float z_near, nb_camera *out);
+nb_status sobol_generate(uint32_t rows, uint32_t dims, uint32_t seed, float *out);
+
+// ---- device layout (HBM) --------------------------------------------------------------------
+//
+// Node pairs: one 64-B record per inner BVH node holding both children's boxes, so a traversal
+// step is one record fetch.  Root records hold the root box as the left child and an all-NaN
+// right box (the slab test misses NaN boxes).  BLAS and TLAS records share one array.
+//   q0 = (L.min.xyz, L.max.x)  q1 = (L.max.yz, R.min.xy)  q2 = (R.min.z, R.max.xyz)
+//   q3 = (L.ref, R.ref, is_root, 0)  as bits
+// A child reference is a record id (bit 31 clear) or a leaf: bit 31 set, primitive count in
+// bits 26..30 (0 = more than 31: the count sits in the leaf-count side table at the leaf's
+// start), first primitive in bits 0..25.
+constexpr uint32_t kLeafBit = 0x80000000u;
+constexpr uint32_t kLeafStartBits = 26;
+constexpr uint32_t kLeafStartMask = (1u << kLeafStartBits) - 1u;
+constexpr uint32_t kLeafMaxInline = 31u;
+constexpr uint32_t kNoRecord = 0xFFFFFFFFu;
+constexpr uint32_t kMaxTreeDepth = 32;  // reference MAX_DEPTH (bvh.rs:153)
+constexpr uint32_t kStackEntries = 2 * kMaxTreeDepth;
+
+// Triangles in BVH (permuted) order, 48 B: (v0.xyz, i0) (e1.xyz, i1) (e2.xyz, i2) with
+// e1 = v1 - v0, e2 = v2 - v0 computed in binary32 exactly as the kernel would.
+// Instance traversal record, 64 B: world_to_local rows 0..2 as (m[0][r], m[1][r], m[2][r], m[3][r])
+// and (root record id, 0, 0, 0).
+// Instance shading record, 96 B: (W2L col0.xyz, roughness) (W2L col1.xyz, metallic)
+// (W2L col2.xyz, emissive) (L2W col0.xyz, color.r) (L2W col1.xyz, color.g) (L2W col2.xyz, color.b).
+// Vertex shading record, 32 B: (normal.xyz, 0) (tangent.xyzw).
+struct DevScene {
+    const float4 *pairs;
+    const float4 *tris;
+    const float4 *inst;
+    const float4 *ishade;
+    const float4 *vshade;
+    const uint32_t *tri_leaf_count;
+    const uint32_t *inst_leaf_count;
+    uint32_t tlas_root;  // record id, or kNoRecord for an empty TLAS
+    uint32_t n_pairs;
+};
+
+struct DevEnv {
+    const float4 *texels;  // 6 faces of (n+2)^2 texels, one texel of seamless border around each face
+    uint32_t n;
+};
+
+struct RenderParams {
+    DevScene scene;
+    DevEnv env;
+    float4 *target;
+    uint32_t width, height;
+    float c2w[16];
+    const float4 *sobol;
+    uint32_t sobol_entries, sobol_stride;
+    uint32_t first_sample, n_frames;
+    float first_weight;      // the caller's weight for the first frame (nb_render)
+    uint32_t weights_from_sample;  // 1: every frame uses 1/sample (nb_render_frames)
+    uint32_t bounces;
+    float contribution_factor;
+    unsigned long long *counters;  // rays, n_aabb, n_tri, n_inst, samples
+    uint32_t tiles_x, n_tiles, tiles_per_xcd;
+    uint32_t *hits_out;      // primary-hit debug output (5 words per pixel) or null
+};
+
+// Kernel launchers (trace.hip).
+nb_status launch_render(const RenderParams &p, void *stream);
+nb_status launch_primary_hits(const RenderParams &p, void *stream);
+
+}  // namespace nb
