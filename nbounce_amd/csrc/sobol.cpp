@@ -88,7 +88,7 @@ nb_status sobol_generate(uint32_t rows, uint32_t dims, uint32_t seed, float *out
             const uint32_t idx = owen(i, s);
             float *o = out + 4 * (static_cast<size_t>(i) * dims + set);
             for (uint32_t k = 0; k < 4; ++k) {
-                uint32_t x = owen(sobol_bits(idx, static_cast<int>(4 * set + k)), combine(s, k + 1));
+                uint32_t x = owen(sobol_bits(idx, static_cast<int>(4 * set + k)), mix32(combine(s, k + 1)));
                 o[k] = static_cast<float>(x >> 8) * (1.0f / 16777216.0f);
             }
         }

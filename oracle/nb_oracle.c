@@ -798,7 +798,7 @@ void orc_sobol_sample4d(uint32_t index, uint32_t dim_set, uint32_t seed, float o
     uint32_t shuffled = nested_uniform_scramble(index, s);
     for (uint32_t k = 0; k < 4; ++k) {
         uint32_t v = orc_sobol_unscrambled(shuffled, 4u * dim_set + k);
-        v = nested_uniform_scramble(v, hash_combine(s, k + 1u));
+        v = nested_uniform_scramble(v, hash_u32(hash_combine(s, k + 1u)));
         out[k] = (float)(v >> 8) * (1.0f / 16777216.0f);
     }
 }
