@@ -13,7 +13,8 @@
  *   - A context is externally synchronized (one host thread at a time).  Rendering calls are
  *     asynchronous on the context's HIP stream; nb_target_read() and nb_context_synchronize()
  *     wait for them.
- *   - Destroy functions accept NULL.
+ *   - Destroy functions accept NULL.  Scenes, targets and environment maps keep their context
+ *     alive: nb_context_destroy() takes effect when the last of them is destroyed.
  */
 #ifndef NBOUNCE_H
 #define NBOUNCE_H

@@ -31,7 +31,7 @@ def common_flags():
         f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC",
         "-ffp-contract=off", "-fhip-fp32-correctly-rounded-divide-sqrt", "-fno-gpu-flush-denormals-to-zero",
         "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
-        "-Wall", "-Wno-unused-function",
+        "-Wall", "-Wno-unused-function", "-Wno-unused-value", "-Wno-unused-result",
         "-I", os.path.join(ROOT, "include"), "-I", CSRC,
     ]
 

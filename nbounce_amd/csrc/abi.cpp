@@ -48,6 +48,10 @@ struct nb_context {
     float4 *sobol = nullptr;
     uint32_t sobol_entries = 0, sobol_stride = 17;
     unsigned long long *counters = nullptr;
+    // Scenes, targets and environments keep their context alive: nb_context_destroy only marks
+    // it, and the last child's destroy releases it.
+    uint32_t children = 0;
+    bool closing = false;
 };
 
 struct nb_scene {
@@ -342,6 +346,8 @@ nb_status timed_launch(nb_context *ctx, const RenderParams &p, nb_status (*launc
 
 }  // namespace
 
+static void release_context(nb_context *ctx);
+
 extern "C" {
 
 const char *nb_last_error(void) { return g_error.c_str(); }
@@ -365,15 +371,15 @@ nb_status nb_context_create(int device, nb_context **out) {
     if (e == hipSuccess) e = hipMalloc(&c->counters, 8 * sizeof(unsigned long long));
     if (e == hipSuccess) e = hipMemset(c->counters, 0, 8 * sizeof(unsigned long long));
     if (e != hipSuccess) {
-        nb_context_destroy(c);
+        c->closing = true;
+        release_context(c);
         return set_error(NB_ERR_HIP, "context creation: %s", hipGetErrorString(e));
     }
     *out = c;
     return NB_OK;
 }
 
-nb_status nb_context_destroy(nb_context *ctx) {
-    if (!ctx) return NB_OK;
+static void release_context(nb_context *ctx) {
     hipSetDevice(ctx->device);
     if (ctx->stream) hipStreamSynchronize(ctx->stream);
     hipFree(ctx->sobol);
@@ -384,6 +390,17 @@ nb_status nb_context_destroy(nb_context *ctx) {
     if (ctx->ev_region_stop) hipEventDestroy(ctx->ev_region_stop);
     if (ctx->stream) hipStreamDestroy(ctx->stream);
     delete ctx;
+}
+
+static void child_released(nb_context *ctx) {
+    if (--ctx->children == 0 && ctx->closing) release_context(ctx);
+}
+
+nb_status nb_context_destroy(nb_context *ctx) {
+    if (!ctx) return NB_OK;
+    if (ctx->closing) return NB_OK;
+    ctx->closing = true;
+    if (ctx->children == 0) release_context(ctx);
     return NB_OK;
 }
 
@@ -464,6 +481,7 @@ nb_status nb_scene_create(nb_context *ctx, const nb_bvh_node *blas, uint32_t n_b
     s->dev.inst_leaf_count = reinterpret_cast<const uint32_t *>(b + off[6]);
     s->dev.tlas_root = R.tlas_root;
     s->dev.n_pairs = static_cast<uint32_t>(R.pairs.size() / 4);
+    ++ctx->children;
     *out = s;
     return NB_OK;
 }
@@ -473,7 +491,9 @@ nb_status nb_scene_destroy(nb_scene *scene) {
     hipSetDevice(scene->ctx->device);
     hipStreamSynchronize(scene->ctx->stream);
     hipFree(scene->block);
+    nb_context *ctx = scene->ctx;
     delete scene;
+    child_released(ctx);
     return NB_OK;
 }
 
@@ -522,6 +542,7 @@ nb_status nb_envmap_create_cube(nb_context *ctx, const float *texels, uint32_t f
         delete m;
         return set_error(NB_ERR_HIP, "environment upload: %s", hipGetErrorString(e));
     }
+    ++ctx->children;
     *out = m;
     return NB_OK;
 }
@@ -531,7 +552,9 @@ nb_status nb_envmap_destroy(nb_envmap *env) {
     hipSetDevice(env->ctx->device);
     hipStreamSynchronize(env->ctx->stream);
     hipFree(env->texels);
+    nb_context *ctx = env->ctx;
     delete env;
+    child_released(ctx);
     return NB_OK;
 }
 
@@ -561,6 +584,7 @@ nb_status nb_target_create(nb_context *ctx, uint32_t width, uint32_t height, nb_
         return set_error(e == hipErrorOutOfMemory ? NB_ERR_OUT_OF_MEMORY : NB_ERR_HIP, "target allocation: %s",
                          hipGetErrorString(e));
     }
+    ++ctx->children;
     *out = t;
     return NB_OK;
 }
@@ -570,7 +594,9 @@ nb_status nb_target_destroy(nb_target *t) {
     hipSetDevice(t->ctx->device);
     hipStreamSynchronize(t->ctx->stream);
     hipFree(t->pixels);
+    nb_context *ctx = t->ctx;
     delete t;
+    child_released(ctx);
     return NB_OK;
 }
 
