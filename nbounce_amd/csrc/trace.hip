@@ -16,6 +16,8 @@
 // which prunes more than the reference's fresh per-instance search but selects the same hit.
 #include "nb_internal.h"
 
+#include <cstdlib>
+
 namespace nb {
 namespace {
 
@@ -504,24 +506,45 @@ __global__ __launch_bounds__(64) void primary_hits_kernel(RenderParams p) {
     o[4] = __float_as_uint(h.t);
 }
 
-constexpr int kLdsEntries = 16;
-
 }  // namespace
 
-nb_status launch_render(const RenderParams &p, void *stream) {
+// Entries of each lane's traversal stack held in LDS (the rest spill to scratch): 16 by default,
+// NB_LDS_STACK=4|8|16|32 selects another instantiation.
+static int lds_entries() {
+    static const int k = [] {
+        const char *e = std::getenv("NB_LDS_STACK");
+        int v = e ? std::atoi(e) : 16;
+        return (v == 4 || v == 8 || v == 16 || v == 32) ? v : 16;
+    }();
+    return k;
+}
+
+template <template <int> class Kern>
+static nb_status launch_with_k(const RenderParams &p, void *stream, const char *what) {
     const uint32_t grid = 8u * p.tiles_per_xcd;
-    hipLaunchKernelGGL(render_kernel<kLdsEntries>, dim3(grid), dim3(64), 0, static_cast<hipStream_t>(stream), p);
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    switch (lds_entries()) {
+    case 4: hipLaunchKernelGGL(Kern<4>::fn, dim3(grid), dim3(64), 0, s, p); break;
+    case 8: hipLaunchKernelGGL(Kern<8>::fn, dim3(grid), dim3(64), 0, s, p); break;
+    case 32: hipLaunchKernelGGL(Kern<32>::fn, dim3(grid), dim3(64), 0, s, p); break;
+    default: hipLaunchKernelGGL(Kern<16>::fn, dim3(grid), dim3(64), 0, s, p); break;
+    }
     hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return set_error(NB_ERR_HIP, "render launch: %s", hipGetErrorString(e));
+    if (e != hipSuccess) return set_error(NB_ERR_HIP, "%s launch: %s", what, hipGetErrorString(e));
     return NB_OK;
 }
 
+namespace {
+template <int K> struct RenderKernel { static constexpr auto fn = render_kernel<K>; };
+template <int K> struct HitsKernel { static constexpr auto fn = primary_hits_kernel<K>; };
+}  // namespace
+
+nb_status launch_render(const RenderParams &p, void *stream) {
+    return launch_with_k<RenderKernel>(p, stream, "render");
+}
+
 nb_status launch_primary_hits(const RenderParams &p, void *stream) {
-    const uint32_t grid = 8u * p.tiles_per_xcd;
-    hipLaunchKernelGGL(primary_hits_kernel<kLdsEntries>, dim3(grid), dim3(64), 0, static_cast<hipStream_t>(stream), p);
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return set_error(NB_ERR_HIP, "primary-hit launch: %s", hipGetErrorString(e));
-    return NB_OK;
+    return launch_with_k<HitsKernel>(p, stream, "primary-hit");
 }
 
 }  // namespace nb
