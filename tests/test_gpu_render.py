@@ -156,22 +156,30 @@ def test_large_scene_frames_match_oracle_at_12_bounces(nb, ctx, large_gpu):
     assert W.rel_err(got, img).max() <= REL_TOL
 
 
-def test_shared_blas_renders_like_the_reference_layout(nb, ctx, large_gpu):
-    """Sharing one BLAS per mesh changes only memory layout: the rendered image is identical to
-    the per-instance BLAS the reference builds (checked where the reference's rebuilt trees
-    agree with the first build, i.e. everywhere for this scene)."""
+def test_shared_blas_layout_matches_oracle_and_reference_layout(nb, ctx, large_gpu):
+    """Sharing one BLAS per mesh is a layout choice: the GPU image on the shared arrays equals the
+    oracle's on the same arrays, and nearly every pixel equals the per-instance layout the
+    reference builds.  Pixels may differ where the reference's own rebuilds leave the first
+    instance of a mesh with a tree whose leaves index re-permuted triangles (scene.rs:285-305)."""
     w, ref, shr, env, arrays = large_gpu
-    cam = _large_camera(nb, 256, 144)
-    print("case: shared vs per-instance BLAS 256x144, 4 frames", flush=True)
-    a = nb.Pathtracer(ctx, ref, cam, env, 256, 144, resolution_factor=1.0, table=W.table(), sobol_stride=17)
-    b = nb.Pathtracer(ctx, shr, cam, env, 256, 144, resolution_factor=1.0, table=W.table(), sobol_stride=17)
+    cam = _large_camera(nb, 128, 72)
+    print("case: shared vs per-instance BLAS 128x72, 4 frames", flush=True)
+    a = nb.Pathtracer(ctx, ref, cam, env, 128, 72, resolution_factor=1.0, table=W.table(), sobol_stride=17)
+    b = nb.Pathtracer(ctx, shr, cam, env, 128, 72, resolution_factor=1.0, table=W.table(), sobol_stride=17)
     for p in (a, b):
         p.globals.bounces = 12
         p.dispatch_frames(4)
     ia, ib = a.output_texture(), b.output_texture()
+    sb_, st_, si_, idx = arrays["shared"]
+    osc, oenv = oracle.Scene(sb_, st_, si_, w["vertices"], idx), oracle.Env(env.faces)
+    img = np.zeros((72, 128, 4), dtype=np.float32)
+    for k in range(1, 5):
+        img, _ = oracle.render(osc, cam.data(), oracle.globals_(k, np.float32(1) / np.float32(k), 12, 4.0), W.table(),
+                               17, oenv, 128, 72, rgba=img, threads=8)
+    assert W.rel_err(ib, img).max() <= REL_TOL
     same = (ia.view(np.uint32) == ib.view(np.uint32)).all(-1).mean()
-    print(f"bit-identical pixels {same:.6f}")
-    assert W.rel_err(ib, ia).max() <= REL_TOL
+    print(f"shared vs per-instance layout: bit-identical pixels {same:.6f}")
+    assert same > 0.95
 
 
 def test_pathtracer_surface_caps_and_invalidates(nb, ctx, small_gpu):
