@@ -88,6 +88,20 @@ def cpu_baseline(P, scene, arrays, env_faces, cam, table, target_s=15.0):
             "sample": f"C oracle, {what} ({rays} rays in {dt:.1f} s, {threads} OpenMP threads)"}
 
 
+def calibration_copy(mib):
+    """One device-to-device copy of `mib` MiB through the HIP runtime (its copy kernel), so that
+    rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE can be scaled against a kernel of known traffic."""
+    import ctypes
+    hip = ctypes.CDLL("libamdhip64.so")
+    n = ctypes.c_size_t(mib << 20)
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    assert hip.hipMalloc(ctypes.byref(a), n) == 0 and hip.hipMalloc(ctypes.byref(b), n) == 0
+    assert hip.hipMemcpy(b, a, n, 3) == 0          # hipMemcpyDeviceToDevice
+    assert hip.hipDeviceSynchronize() == 0
+    hip.hipFree(a)
+    hip.hipFree(b)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -99,6 +113,8 @@ def main():
     ap.add_argument("--reference-blas", action="store_true", help="rebuild the BLAS per instance as the reference")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--calibrate-copy-mib", type=int, default=0,
+                    help="also run one device copy of this many MiB (counter calibration under rocprofv3)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported "
@@ -129,6 +145,8 @@ def main():
             else:
                 done += pt.dispatch_frames(min(F, n_steps - done))
 
+    if args.calibrate_copy_mib:
+        calibration_copy(args.calibrate_copy_mib)
     step_block(args.warmup)
     ctx.synchronize()
     pt.invalidate()
