@@ -102,6 +102,62 @@ def calibration_copy(mib):
     hip.hipFree(b)
 
 
+def serve_bench(args):
+    """Serving workload: 4 resident instanced scenes (seeds 0-3 with 1024..4096 instances),
+    Poisson arrivals at --qps, resolution from {256^2, 512^2, 1024x768, 1920x1080} with
+    p = 0.4/0.3/0.2/0.1, spp uniform over {1, 4, 16, 64}, random cameras, 8 bounces, HBM
+    admission budget 200 GB.  Reports request latency p50/p99 and aggregate Mrays/s."""
+    import nbounce_amd as nb
+    from nbounce_amd import scenegen, serving
+    ctx = nb.Context(0)
+    table = nb.sobol_table(1024, 17, 0)
+    from nbounce_amd._lib import check, lib, ptr
+    check(lib.nb_sobol_set(ctx.handle, ptr(table), 1024, 17, 17))
+    srv = serving.Server(ctx, hbm_budget_bytes=200 << 30, max_paths=args.max_paths, max_bounces=12)
+    env = nb.EnvMap(ctx, scenegen.sky_cube(256))
+    keep = []
+    for seed, n_inst in zip(range(4), (1024, 2048, 3072, 4096)):
+        v, i, pr = scenegen.instanced_meshes(seed, n_instances=n_inst)
+        sc = nb.Scene(v, i, pr)
+        b, t, inst = nb.build_scene(sc, share_blas=True)
+        sb = nb.SceneBuffers(ctx, b, t, inst, sc.vertices, sc.indices)
+        srv.add_scene(sb, env)
+        keep.append(sb)
+    reqs = serving.mixed_requests(args.requests, args.qps, 4, seed=0)
+
+    def camera(r):
+        return nb.CameraController(world_position=r["eye"], target=r["target"],
+                                   aspect_ratio=r["width"] / r["height"]).data()
+
+    # warm-up: two small requests so first-use costs stay out of the timed run
+    serving.run_open_loop(srv, [dict(r, arrival=0.0) for r in reqs[:2]], camera)
+    s0 = srv.stats()
+    lat, wall, _ = serving.run_open_loop(srv, reqs, camera)
+    s1 = srv.stats()
+    rays = s1["rays"] - s0["rays"]
+    out = {
+        "metric": "serving request latency p50/p99 and aggregate Mrays/s",
+        "value": round(float(np.percentile(lat, 99)) * 1e3, 2),
+        "unit": "ms (p99 latency)",
+        "higher_is_better": False,
+        "p50_ms": round(float(np.percentile(lat, 50)) * 1e3, 2),
+        "p99_ms": round(float(np.percentile(lat, 99)) * 1e3, 2),
+        "mean_ms": round(float(lat.mean()) * 1e3, 2),
+        "aggregate_mrays_s": round(rays / wall / 1e6, 2),
+        "gpu_busy_frac": round((s1["gpu_busy_s"] - s0["gpu_busy_s"]) / wall, 3),
+        "requests": args.requests,
+        "qps": args.qps,
+        "offered_samples_per_s_M": round(sum(r["width"] * r["height"] * r["spp"] for r in reqs) / reqs[-1]["arrival"] / 1e6, 2),
+        "steps": s1["steps"] - s0["steps"],
+        "wall_s": round(wall, 3),
+        "n_gpus": 1,
+        "config": {"workload": "4 resident instanced scenes (1024-4096 instances), Poisson arrivals, mixed "
+                               "resolution and spp, 8 bounces", "max_paths_per_step": args.max_paths,
+                   "hbm_budget_gb": 200},
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -113,6 +169,10 @@ def main():
     ap.add_argument("--reference-blas", action="store_true", help="rebuild the BLAS per instance as the reference")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--mode", choices=["render", "serve"], default="render")
+    ap.add_argument("--qps", type=float, default=20.0)
+    ap.add_argument("--requests", type=int, default=200)
+    ap.add_argument("--max-paths", type=int, default=1 << 23)
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="also run one device copy of this many MiB (counter calibration under rocprofv3)")
     args = ap.parse_args()
@@ -120,6 +180,9 @@ def main():
         print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported "
               "(the path shards by image tiles or Sobol rows, which this build does not run)", file=sys.stderr)
         sys.exit(2)
+    if args.mode == "serve":
+        serve_bench(args)
+        return
 
     import nbounce_amd as nb
     from nbounce_amd import scenegen

@@ -173,6 +173,51 @@ nb_status nb_counters_reset(nb_context *ctx);
 nb_status nb_primary_hits(nb_context *ctx, const nb_scene *scene, const nb_camera *camera,
                           uint32_t width, uint32_t height, uint32_t sample, uint32_t *out);
 
+/* ---- serving: continuous batching over resident scenes -------------------------------------
+ * No counterpart in the reference (its frame loop, app.rs:177-219, renders one scene for one
+ * window).  A server keeps several scenes resident, admits render requests while earlier ones
+ * are on the GPU (as request slots and HBM budget free up), and renders every admitted request's
+ * samples together in steps: one captured HIP graph per step (path generation, one persistent
+ * wavefront kernel per bounce level with wave64 ballot compaction, ordered accumulation).  Each
+ * request's image equals what nb_render_frames produces for it alone (same camera, size,
+ * samples 1..spp, bounces, contribution factor, the context's Sobol table).  A server is used
+ * from one host thread at a time. */
+typedef struct nb_server nb_server;
+
+typedef struct {
+    nb_camera camera;
+    uint32_t scene;                 /* index from nb_server_add_scene */
+    uint32_t width, height;         /* nonzero multiples of 8 */
+    uint32_t spp;                   /* samples 1..spp */
+    uint32_t bounces;
+    float contribution_factor;
+    double arrival_s;               /* server clock; negative = the time of submission */
+} nb_request;
+
+typedef struct { uint64_t id; double arrival_s, admit_s, done_s; } nb_completion;
+
+typedef struct {
+    uint64_t steps, rays, samples, n_aabb, n_tri, n_inst, hits;
+    uint64_t bytes_in_use, bytes_budget;
+    uint32_t active, queued, completed, pad;
+    double gpu_busy_s;              /* summed device time of the steps */
+} nb_server_stats;
+
+nb_status nb_server_create(nb_context *ctx, uint64_t hbm_budget_bytes, uint32_t max_paths_per_step,
+                           uint32_t max_bounces, nb_server **out);
+nb_status nb_server_destroy(nb_server *srv);
+nb_status nb_server_add_scene(nb_server *srv, const nb_scene *scene, const nb_envmap *env, uint32_t *index);
+nb_status nb_server_submit(nb_server *srv, const nb_request *req, uint64_t *id);
+/* Admits what fits, renders one step of the admitted requests and waits for it. */
+nb_status nb_server_step(nb_server *srv, uint32_t *n_active, uint32_t *n_queued);
+/* Completed requests since the last poll (up to cap). */
+nb_status nb_server_poll(nb_server *srv, nb_completion *out, uint32_t cap, uint32_t *n);
+nb_status nb_server_read(nb_server *srv, uint64_t id, float *rgba);
+/* Frees a completed request's image (its HBM returns to the admission budget). */
+nb_status nb_server_release(nb_server *srv, uint64_t id);
+nb_status nb_server_now(nb_server *srv, double *seconds);
+nb_status nb_server_stats_get(nb_server *srv, nb_server_stats *out);
+
 #ifdef __cplusplus
 }
 #endif

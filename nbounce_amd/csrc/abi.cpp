@@ -39,40 +39,6 @@ nb_status set_error(nb_status st, const char *fmt, ...) {
 
 using namespace nb;
 
-struct nb_context {
-    int device = 0;
-    hipStream_t stream = nullptr;
-    hipEvent_t ev_start = nullptr, ev_stop = nullptr;
-    hipEvent_t ev_region_start = nullptr, ev_region_stop = nullptr;
-    bool timed = false, region_started = false;
-    float4 *sobol = nullptr;
-    uint32_t sobol_entries = 0, sobol_stride = 17;
-    unsigned long long *counters = nullptr;
-    // Scenes, targets and environments keep their context alive: nb_context_destroy only marks
-    // it, and the last child's destroy releases it.
-    uint32_t children = 0;
-    bool closing = false;
-};
-
-struct nb_scene {
-    nb_context *ctx = nullptr;
-    void *block = nullptr;
-    uint64_t bytes = 0;
-    DevScene dev{};
-};
-
-struct nb_target {
-    nb_context *ctx = nullptr;
-    float4 *pixels = nullptr;
-    uint32_t width = 0, height = 0;
-};
-
-struct nb_envmap {
-    nb_context *ctx = nullptr;
-    float4 *texels = nullptr;
-    uint32_t n = 0;
-};
-
 namespace {
 
 nb_status use_device(const nb_context *ctx) {
@@ -396,6 +362,7 @@ static void child_released(nb_context *ctx) {
     if (--ctx->children == 0 && ctx->closing) release_context(ctx);
 }
 
+
 nb_status nb_context_destroy(nb_context *ctx) {
     if (!ctx) return NB_OK;
     if (ctx->closing) return NB_OK;
@@ -718,3 +685,8 @@ nb_status nb_primary_hits(nb_context *ctx, const nb_scene *scene, const nb_camer
 }
 
 }  // extern "C"
+
+namespace nb {
+void context_add_child(nb_context *ctx) { ++ctx->children; }
+void context_release_child(nb_context *ctx) { child_released(ctx); }
+}  // namespace nb

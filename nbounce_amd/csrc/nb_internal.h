@@ -82,8 +82,96 @@ struct RenderParams {
     uint32_t *hits_out;      // primary-hit debug output (5 words per pixel) or null
 };
 
+// ---- serving (serve.hip, server.cpp) ----------------------------------------------------------
+struct ServeSceneDev {
+    DevScene scene;
+    DevEnv env;
+};
+
+// One admitted request, indexed by its slot.
+struct ServeRequestDev {
+    float c2w[16];
+    float4 *target;          // RGBA32F output, zeroed at admission
+    uint32_t width, height;
+    uint32_t scene;          // scene slot
+    uint32_t bounces;
+    float contribution_factor;
+    uint32_t pad[3];
+};
+
+// The samples one request renders in one step: samples first_sample .. first_sample + count - 1
+// of every pixel.  Its paths occupy ids [path_base, path_base + count * pixels), sample-major;
+// its pixels occupy [pixel_base, pixel_base + pixels) of the step's accumulation range.
+struct ServePass {
+    uint32_t slot, first_sample, count, pixels;
+    uint32_t path_base, pixel_base, pad0, pad1;
+};
+
+// Per-step description, at a fixed device address so that one captured graph serves every step.
+struct ServeStep {
+    const ServeSceneDev *scenes;
+    const ServeRequestDev *requests;
+    const ServePass *passes;
+    uint32_t n_passes, n_paths, n_pixels, max_bounces;
+    const float4 *sobol;
+    uint32_t sobol_entries, sobol_stride;
+    float4 *queue[2];        // path records, 64 B each, ping-pong between bounce levels
+    float4 *results;         // radiance per path id
+    uint32_t *level;         // per bounce level: [2*l] = paths queued, [2*l+1] = claimed
+    unsigned long long *counters;
+};
+
+constexpr uint32_t kServeMaxBounces = 32;
+
+nb_status serve_launch_reset(const ServeStep *step, uint32_t grid, hipStream_t s);
+nb_status serve_launch_generate(const ServeStep *step, uint32_t grid, hipStream_t s);
+nb_status serve_launch_bounce(const ServeStep *step, uint32_t level, uint32_t grid, hipStream_t s);
+nb_status serve_launch_accumulate(const ServeStep *step, uint32_t grid, hipStream_t s);
+
 // Kernel launchers (trace.hip).
 nb_status launch_render(const RenderParams &p, void *stream);
 nb_status launch_primary_hits(const RenderParams &p, void *stream);
 
+}  // namespace nb
+
+// ---- opaque handles of the C ABI ---------------------------------------------------------------
+struct nb_context {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev_start = nullptr, ev_stop = nullptr;
+    hipEvent_t ev_region_start = nullptr, ev_region_stop = nullptr;
+    bool timed = false, region_started = false;
+    float4 *sobol = nullptr;
+    uint32_t sobol_entries = 0, sobol_stride = 17;
+    unsigned long long *counters = nullptr;
+    // Scenes, targets and environments keep their context alive: nb_context_destroy only marks
+    // it, and the last child's destroy releases it.
+    uint32_t children = 0;
+    bool closing = false;
+};
+
+struct nb_scene {
+    nb_context *ctx = nullptr;
+    void *block = nullptr;
+    uint64_t bytes = 0;
+    nb::DevScene dev{};
+};
+
+struct nb_target {
+    nb_context *ctx = nullptr;
+    float4 *pixels = nullptr;
+    uint32_t width = 0, height = 0;
+};
+
+struct nb_envmap {
+    nb_context *ctx = nullptr;
+    float4 *texels = nullptr;
+    uint32_t n = 0;
+};
+
+
+namespace nb {
+// A scene, target, environment or server created on a context keeps it alive.
+void context_add_child(nb_context *ctx);
+void context_release_child(nb_context *ctx);
 }  // namespace nb

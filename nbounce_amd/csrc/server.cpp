@@ -1,0 +1,387 @@
+// Serving host side (nb_server_* in include/nbounce.h): resident scenes, request admission by
+// slot and HBM budget, continuous batching of every admitted request's samples into steps, and
+// one captured HIP graph that renders a step.
+//
+// Scheduling: admitted requests are visited round-robin in admission order; each round gives a
+// request one more sample pass (all its pixels) while the step's path budget lasts, so small
+// requests finish in a step or two even while a large one is being rendered, and a request that
+// arrives while a step runs joins the next step.
+#include "nbounce.h"
+#include "nb_internal.h"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <vector>
+
+using namespace nb;
+
+namespace {
+
+#define SRV_HIP(call)                                                                                  \
+    do {                                                                                               \
+        hipError_t e_ = (call);                                                                        \
+        if (e_ != hipSuccess)                                                                          \
+            return set_error(e_ == hipErrorOutOfMemory ? NB_ERR_OUT_OF_MEMORY : NB_ERR_HIP, "%s: %s", \
+                             #call, hipGetErrorString(e_));                                            \
+    } while (0)
+
+#define SRV_TRY(expr)                  \
+    do {                               \
+        nb_status s_ = (expr);         \
+        if (s_ != NB_OK) return s_;    \
+    } while (0)
+
+constexpr uint32_t kMaxSlots = 256;
+constexpr uint32_t kMaxScenes = 64;
+
+struct Request {
+    uint64_t id = 0;
+    nb_request req{};
+    double arrival = 0, admit = -1, done = -1;
+    uint32_t next_sample = 1;     // next sample to schedule
+    int slot = -1;
+    float4 *target = nullptr;
+    uint64_t bytes = 0;
+};
+
+}  // namespace
+
+struct nb_server {
+    nb_context *ctx = nullptr;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    uint64_t budget = 0, in_use = 0;
+    uint32_t max_paths = 0, max_bounces = 0, cu = 0;
+    std::chrono::steady_clock::time_point t0;
+
+    // device buffers
+    float4 *queue[2] = {nullptr, nullptr};
+    float4 *results = nullptr;
+    uint32_t *level = nullptr;
+    unsigned long long *counters = nullptr;
+    ServeStep *d_step = nullptr;
+    ServePass *d_passes = nullptr;
+    ServeRequestDev *d_requests = nullptr;
+    ServeSceneDev *d_scenes = nullptr;
+    // pinned staging
+    ServeStep *h_step = nullptr;
+    ServePass *h_passes = nullptr;
+    ServeRequestDev *h_requests = nullptr;
+    ServeSceneDev *h_scenes = nullptr;
+    uint32_t n_scenes = 0;
+
+    hipGraphExec_t graph = nullptr;
+    uint64_t next_id = 1;
+    std::deque<Request *> queued;
+    std::vector<Request *> active;              // admission order
+    Request *slots[kMaxSlots] = {};
+    std::map<uint64_t, Request *> finished;     // completed, image still held
+    std::vector<nb_completion> unpolled;
+    uint64_t steps = 0;
+    double busy = 0;
+    uint32_t completed = 0;
+
+    double now() const {
+        return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    }
+};
+
+namespace {
+
+nb_status capture_graph(nb_server *s) {
+    const uint32_t grid_stream = s->cu * 8, grid_bounce = s->cu * 16;
+    SRV_HIP(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
+    nb_status st = serve_launch_reset(s->d_step, 1, s->stream);
+    if (st == NB_OK) st = serve_launch_generate(s->d_step, grid_stream, s->stream);
+    for (uint32_t l = 0; st == NB_OK && l <= s->max_bounces; ++l) st = serve_launch_bounce(s->d_step, l, grid_bounce, s->stream);
+    if (st == NB_OK) st = serve_launch_accumulate(s->d_step, grid_stream, s->stream);
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(s->stream, &g);
+    if (st != NB_OK) { if (g) hipGraphDestroy(g); return st; }
+    if (e != hipSuccess) return set_error(NB_ERR_HIP, "graph capture: %s", hipGetErrorString(e));
+    e = hipGraphInstantiate(&s->graph, g, nullptr, nullptr, 0);
+    hipGraphDestroy(g);
+    if (e != hipSuccess) return set_error(NB_ERR_HIP, "graph instantiate: %s", hipGetErrorString(e));
+    return NB_OK;
+}
+
+void free_server(nb_server *s) {
+    hipSetDevice(s->ctx->device);
+    if (s->stream) hipStreamSynchronize(s->stream);
+    if (s->graph) hipGraphExecDestroy(s->graph);
+    for (Request *r : s->queued) delete r;
+    for (Request *r : s->active) { hipFree(r->target); delete r; }
+    for (auto &kv : s->finished) { hipFree(kv.second->target); delete kv.second; }
+    hipFree(s->queue[0]); hipFree(s->queue[1]); hipFree(s->results); hipFree(s->level); hipFree(s->counters);
+    hipFree(s->d_step); hipFree(s->d_passes); hipFree(s->d_requests); hipFree(s->d_scenes);
+    hipHostFree(s->h_step); hipHostFree(s->h_passes); hipHostFree(s->h_requests); hipHostFree(s->h_scenes);
+    if (s->ev0) hipEventDestroy(s->ev0);
+    if (s->ev1) hipEventDestroy(s->ev1);
+    if (s->stream) hipStreamDestroy(s->stream);
+    nb_context *ctx = s->ctx;
+    delete s;
+    context_release_child(ctx);
+}
+
+nb_status admit(nb_server *s) {
+    while (!s->queued.empty()) {
+        Request *r = s->queued.front();
+        int slot = -1;
+        for (uint32_t k = 0; k < kMaxSlots; ++k) if (!s->slots[k]) { slot = static_cast<int>(k); break; }
+        if (slot < 0 || s->in_use + r->bytes > s->budget) break;   // FIFO: wait for slots / memory
+        SRV_HIP(hipMalloc(&r->target, r->bytes));
+        SRV_HIP(hipMemsetAsync(r->target, 0, r->bytes, s->stream));
+        s->in_use += r->bytes;
+        r->slot = slot;
+        r->admit = s->now();
+        s->slots[slot] = r;
+        s->active.push_back(r);
+        s->queued.pop_front();
+        ServeRequestDev &d = s->h_requests[slot];
+        std::memcpy(d.c2w, r->req.camera.clip_to_world, sizeof(d.c2w));
+        d.target = r->target;
+        d.width = r->req.width;
+        d.height = r->req.height;
+        d.scene = r->req.scene;
+        d.bounces = r->req.bounces;
+        d.contribution_factor = r->req.contribution_factor;
+    }
+    return NB_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+nb_status nb_server_create(nb_context *ctx, uint64_t budget, uint32_t max_paths, uint32_t max_bounces, nb_server **out) {
+    if (!ctx || !out) return set_error(NB_ERR_INVALID_ARGUMENT, "null argument");
+    *out = nullptr;
+    if (max_paths < 4096 || max_paths > (1u << 26)) return set_error(NB_ERR_INVALID_ARGUMENT, "max_paths must be in 4096..2^26");
+    if (max_bounces > kServeMaxBounces) return set_error(NB_ERR_INVALID_ARGUMENT, "max_bounces must be <= %u", kServeMaxBounces);
+    SRV_HIP(hipSetDevice(ctx->device));
+    nb_server *s = new nb_server();
+    s->ctx = ctx;
+    context_add_child(ctx);
+    s->budget = budget;
+    s->max_paths = max_paths;
+    s->max_bounces = max_bounces;
+    s->t0 = std::chrono::steady_clock::now();
+    int cu = 0;
+    hipDeviceGetAttribute(&cu, hipDeviceAttributeMultiprocessorCount, ctx->device);
+    s->cu = cu > 0 ? static_cast<uint32_t>(cu) : 64u;
+    const size_t qbytes = static_cast<size_t>(max_paths) * 64;
+    hipError_t e = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreate(&s->ev0);
+    if (e == hipSuccess) e = hipEventCreate(&s->ev1);
+    if (e == hipSuccess) e = hipMalloc(&s->queue[0], qbytes);
+    if (e == hipSuccess) e = hipMalloc(&s->queue[1], qbytes);
+    if (e == hipSuccess) e = hipMalloc(&s->results, static_cast<size_t>(max_paths) * 16);
+    if (e == hipSuccess) e = hipMalloc(&s->level, 2 * (kServeMaxBounces + 2) * sizeof(uint32_t));
+    if (e == hipSuccess) e = hipMalloc(&s->counters, 8 * sizeof(unsigned long long));
+    if (e == hipSuccess) e = hipMemset(s->counters, 0, 8 * sizeof(unsigned long long));
+    if (e == hipSuccess) e = hipMalloc(&s->d_step, sizeof(ServeStep));
+    if (e == hipSuccess) e = hipMalloc(&s->d_passes, kMaxSlots * sizeof(ServePass));
+    if (e == hipSuccess) e = hipMalloc(&s->d_requests, kMaxSlots * sizeof(ServeRequestDev));
+    if (e == hipSuccess) e = hipMalloc(&s->d_scenes, kMaxScenes * sizeof(ServeSceneDev));
+    if (e == hipSuccess) e = hipHostMalloc(&s->h_step, sizeof(ServeStep), hipHostMallocDefault);
+    if (e == hipSuccess) e = hipHostMalloc(&s->h_passes, kMaxSlots * sizeof(ServePass), hipHostMallocDefault);
+    if (e == hipSuccess) e = hipHostMalloc(&s->h_requests, kMaxSlots * sizeof(ServeRequestDev), hipHostMallocDefault);
+    if (e == hipSuccess) e = hipHostMalloc(&s->h_scenes, kMaxScenes * sizeof(ServeSceneDev), hipHostMallocDefault);
+    if (e != hipSuccess) {
+        free_server(s);
+        return set_error(e == hipErrorOutOfMemory ? NB_ERR_OUT_OF_MEMORY : NB_ERR_HIP, "server allocation: %s",
+                         hipGetErrorString(e));
+    }
+    std::memset(s->h_requests, 0, kMaxSlots * sizeof(ServeRequestDev));
+    std::memset(s->h_step, 0, sizeof(ServeStep));
+    nb_status st = capture_graph(s);
+    if (st != NB_OK) { free_server(s); return st; }
+    *out = s;
+    return NB_OK;
+}
+
+nb_status nb_server_destroy(nb_server *s) {
+    if (!s) return NB_OK;
+    free_server(s);
+    return NB_OK;
+}
+
+nb_status nb_server_add_scene(nb_server *s, const nb_scene *scene, const nb_envmap *env, uint32_t *index) {
+    if (!s || !scene || !env || !index) return set_error(NB_ERR_INVALID_ARGUMENT, "null argument");
+    if (scene->ctx != s->ctx || env->ctx != s->ctx) return set_error(NB_ERR_INVALID_ARGUMENT, "scene and environment must belong to the server's context");
+    if (s->n_scenes >= kMaxScenes) return set_error(NB_ERR_UNSUPPORTED, "at most %u resident scenes", kMaxScenes);
+    s->h_scenes[s->n_scenes] = ServeSceneDev{scene->dev, DevEnv{env->texels, env->n}};
+    SRV_HIP(hipSetDevice(s->ctx->device));
+    SRV_HIP(hipMemcpy(s->d_scenes, s->h_scenes, (s->n_scenes + 1) * sizeof(ServeSceneDev), hipMemcpyHostToDevice));
+    *index = s->n_scenes++;
+    return NB_OK;
+}
+
+nb_status nb_server_submit(nb_server *s, const nb_request *req, uint64_t *id) {
+    if (!s || !req || !id) return set_error(NB_ERR_INVALID_ARGUMENT, "null argument");
+    if (req->scene >= s->n_scenes) return set_error(NB_ERR_INVALID_ARGUMENT, "scene %u not added", req->scene);
+    if (!req->width || !req->height || req->width % 8 || req->height % 8)
+        return set_error(NB_ERR_INVALID_ARGUMENT, "size %ux%u: sides must be nonzero multiples of 8", req->width, req->height);
+    const uint64_t pixels = static_cast<uint64_t>(req->width) * req->height;
+    if (pixels > s->max_paths) return set_error(NB_ERR_INVALID_ARGUMENT, "%llu pixels exceed the step budget of %u paths",
+                                                 static_cast<unsigned long long>(pixels), s->max_paths);
+    if (req->spp == 0) return set_error(NB_ERR_INVALID_ARGUMENT, "spp must be >= 1");
+    if (req->bounces > s->max_bounces) return set_error(NB_ERR_INVALID_ARGUMENT, "bounces %u > server maximum %u", req->bounces, s->max_bounces);
+    const uint64_t bytes = pixels * 16;
+    if (bytes > s->budget) return set_error(NB_ERR_BUDGET, "request needs %llu B, more than the HBM budget", static_cast<unsigned long long>(bytes));
+    Request *r = new Request();
+    r->id = s->next_id++;
+    r->req = *req;
+    r->arrival = req->arrival_s >= 0 ? req->arrival_s : s->now();
+    r->bytes = bytes;
+    s->queued.push_back(r);
+    *id = r->id;
+    return NB_OK;
+}
+
+nb_status nb_server_step(nb_server *s, uint32_t *n_active, uint32_t *n_queued) {
+    if (!s) return set_error(NB_ERR_INVALID_ARGUMENT, "null server");
+    if (!s->ctx->sobol) return set_error(NB_ERR_INVALID_ARGUMENT, "no Sobol table set on the context");
+    SRV_HIP(hipSetDevice(s->ctx->device));
+    SRV_TRY(admit(s));
+    // plan: round-robin sample passes within the path budget
+    uint32_t n_passes = 0;
+    uint64_t paths = 0;
+    std::vector<uint32_t> take(s->active.size(), 0);
+    for (bool progress = true; progress;) {
+        progress = false;
+        for (size_t k = 0; k < s->active.size(); ++k) {
+            Request *r = s->active[k];
+            const uint64_t px = static_cast<uint64_t>(r->req.width) * r->req.height;
+            const uint32_t left = r->req.spp - (r->next_sample - 1) - take[k];
+            if (left == 0 || paths + px > s->max_paths) continue;
+            ++take[k];
+            paths += px;
+            progress = true;
+        }
+    }
+    uint64_t pixel_sum = 0;
+    paths = 0;
+    for (size_t k = 0; k < s->active.size(); ++k) {
+        if (!take[k]) continue;
+        Request *r = s->active[k];
+        const uint32_t px = r->req.width * r->req.height;
+        ServePass &p = s->h_passes[n_passes++];
+        p.slot = static_cast<uint32_t>(r->slot);
+        p.first_sample = r->next_sample;
+        p.count = take[k];
+        p.pixels = px;
+        p.path_base = static_cast<uint32_t>(paths);
+        p.pixel_base = static_cast<uint32_t>(pixel_sum);
+        paths += static_cast<uint64_t>(take[k]) * px;
+        pixel_sum += px;
+    }
+    if (n_passes > 0) {
+        ServeStep &st = *s->h_step;
+        st.scenes = s->d_scenes;
+        st.requests = s->d_requests;
+        st.passes = s->d_passes;
+        st.n_passes = n_passes;
+        st.n_paths = static_cast<uint32_t>(paths);
+        st.n_pixels = static_cast<uint32_t>(pixel_sum);
+        st.max_bounces = s->max_bounces;
+        st.sobol = s->ctx->sobol;
+        st.sobol_entries = s->ctx->sobol_entries;
+        st.sobol_stride = s->ctx->sobol_stride;
+        st.queue[0] = s->queue[0];
+        st.queue[1] = s->queue[1];
+        st.results = s->results;
+        st.level = s->level;
+        st.counters = s->counters;
+        SRV_HIP(hipMemcpyAsync(s->d_requests, s->h_requests, kMaxSlots * sizeof(ServeRequestDev), hipMemcpyHostToDevice, s->stream));
+        SRV_HIP(hipMemcpyAsync(s->d_passes, s->h_passes, n_passes * sizeof(ServePass), hipMemcpyHostToDevice, s->stream));
+        SRV_HIP(hipMemcpyAsync(s->d_step, s->h_step, sizeof(ServeStep), hipMemcpyHostToDevice, s->stream));
+        SRV_HIP(hipEventRecord(s->ev0, s->stream));
+        SRV_HIP(hipGraphLaunch(s->graph, s->stream));
+        SRV_HIP(hipEventRecord(s->ev1, s->stream));
+        SRV_HIP(hipEventSynchronize(s->ev1));
+        float ms = 0;
+        hipEventElapsedTime(&ms, s->ev0, s->ev1);
+        s->busy += ms * 1e-3;
+        ++s->steps;
+        const double t = s->now();
+        for (size_t k = 0; k < s->active.size(); ++k) s->active[k]->next_sample += take[k];
+        std::vector<Request *> still;
+        for (Request *r : s->active) {
+            if (r->next_sample > r->req.spp) {
+                r->done = t;
+                s->slots[r->slot] = nullptr;
+                s->finished[r->id] = r;
+                s->unpolled.push_back(nb_completion{r->id, r->arrival, r->admit, r->done});
+                ++s->completed;
+            } else {
+                still.push_back(r);
+            }
+        }
+        s->active.swap(still);
+    }
+    if (n_active) *n_active = static_cast<uint32_t>(s->active.size());
+    if (n_queued) *n_queued = static_cast<uint32_t>(s->queued.size());
+    return NB_OK;
+}
+
+nb_status nb_server_poll(nb_server *s, nb_completion *out, uint32_t cap, uint32_t *n) {
+    if (!s || !n || (cap && !out)) return set_error(NB_ERR_INVALID_ARGUMENT, "null argument");
+    uint32_t k = std::min<uint32_t>(cap, static_cast<uint32_t>(s->unpolled.size()));
+    for (uint32_t i = 0; i < k; ++i) out[i] = s->unpolled[i];
+    s->unpolled.erase(s->unpolled.begin(), s->unpolled.begin() + k);
+    *n = k;
+    return NB_OK;
+}
+
+nb_status nb_server_read(nb_server *s, uint64_t id, float *rgba) {
+    if (!s || !rgba) return set_error(NB_ERR_INVALID_ARGUMENT, "null argument");
+    auto it = s->finished.find(id);
+    if (it == s->finished.end()) return set_error(NB_ERR_INVALID_ARGUMENT, "request %llu is not completed or was released", static_cast<unsigned long long>(id));
+    SRV_HIP(hipSetDevice(s->ctx->device));
+    SRV_HIP(hipMemcpy(rgba, it->second->target, it->second->bytes, hipMemcpyDeviceToHost));
+    return NB_OK;
+}
+
+nb_status nb_server_release(nb_server *s, uint64_t id) {
+    if (!s) return set_error(NB_ERR_INVALID_ARGUMENT, "null server");
+    auto it = s->finished.find(id);
+    if (it == s->finished.end()) return set_error(NB_ERR_INVALID_ARGUMENT, "request %llu is not completed or was released", static_cast<unsigned long long>(id));
+    SRV_HIP(hipSetDevice(s->ctx->device));
+    hipFree(it->second->target);
+    s->in_use -= it->second->bytes;
+    delete it->second;
+    s->finished.erase(it);
+    return NB_OK;
+}
+
+nb_status nb_server_now(nb_server *s, double *t) {
+    if (!s || !t) return set_error(NB_ERR_INVALID_ARGUMENT, "null argument");
+    *t = s->now();
+    return NB_OK;
+}
+
+nb_status nb_server_stats_get(nb_server *s, nb_server_stats *out) {
+    if (!s || !out) return set_error(NB_ERR_INVALID_ARGUMENT, "null argument");
+    SRV_HIP(hipSetDevice(s->ctx->device));
+    SRV_HIP(hipStreamSynchronize(s->stream));
+    unsigned long long v[8];
+    SRV_HIP(hipMemcpy(v, s->counters, sizeof(v), hipMemcpyDeviceToHost));
+    std::memset(out, 0, sizeof(*out));
+    out->steps = s->steps;
+    out->rays = v[0]; out->n_aabb = v[1]; out->n_tri = v[2]; out->n_inst = v[3]; out->samples = v[4]; out->hits = v[5];
+    out->bytes_in_use = s->in_use;
+    out->bytes_budget = s->budget;
+    out->active = static_cast<uint32_t>(s->active.size());
+    out->queued = static_cast<uint32_t>(s->queued.size());
+    out->completed = s->completed;
+    out->gpu_busy_s = s->busy;
+    return NB_OK;
+}
+
+}  // extern "C"

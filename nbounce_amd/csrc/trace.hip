@@ -14,415 +14,13 @@
 // entering an instance records the stack height, and the BLAS traversal of that instance ends
 // when the stack falls back to it.  The current closest distance is handed into each BLAS,
 // which prunes more than the reference's fresh per-instance search but selects the same hit.
-#include "nb_internal.h"
+#include "pt_device.h"
 
 #include <cstdlib>
 
 namespace nb {
 namespace {
-
-constexpr float kNoHit = 3.40282346638528859812e+38f;  // raytracing_sw.wgsl:1-2
-constexpr float kEps = 0.00000001f;                    // raytracing_sw.wgsl:3
-constexpr float kBias = 0.1f;                          // raytracing_sw.wgsl:4
-constexpr float kPi = 3.14159265359f;                  // common.wgsl:1
-constexpr float kTwoPi = 2.0f * kPi;
-
-struct F3 { float x, y, z; };
-__device__ __forceinline__ F3 f3(float x, float y, float z) { return F3{x, y, z}; }
-__device__ __forceinline__ F3 add(F3 a, F3 b) { return f3(a.x + b.x, a.y + b.y, a.z + b.z); }
-__device__ __forceinline__ F3 sub(F3 a, F3 b) { return f3(a.x - b.x, a.y - b.y, a.z - b.z); }
-__device__ __forceinline__ F3 mul(F3 a, F3 b) { return f3(a.x * b.x, a.y * b.y, a.z * b.z); }
-__device__ __forceinline__ F3 scale(float s, F3 a) { return f3(s * a.x, s * a.y, s * a.z); }
-__device__ __forceinline__ F3 divs(F3 a, float s) { return f3(a.x / s, a.y / s, a.z / s); }
-__device__ __forceinline__ F3 rcp3(F3 a) { return f3(1.0f / a.x, 1.0f / a.y, 1.0f / a.z); }
-__device__ __forceinline__ float dot(F3 a, F3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
-__device__ __forceinline__ F3 cross(F3 a, F3 b) {
-    return f3(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
-}
-__device__ __forceinline__ F3 normalize(F3 v) {
-    float s = 1.0f / sqrtf(dot(v, v));
-    return f3(v.x * s, v.y * s, v.z * s);
-}
-__device__ __forceinline__ float fract(float x) { return x - floorf(x); }
-__device__ __forceinline__ float luminance(F3 c) { return 0.2126f * c.x + 0.7152f * c.y + 0.0722f * c.z; }
-__device__ __forceinline__ float pow5(float x) { float x2 = x * x; float x4 = x2 * x2; return x4 * x; }
-__device__ __forceinline__ float pow8(float x) { float x2 = x * x; float x4 = x2 * x2; return x4 * x4; }
-__device__ __forceinline__ F3 xyz(float4 q) { return f3(q.x, q.y, q.z); }
-
-// sin / cos on binary32: quadrant q = rint(x * 2/pi), r = x - q*pi/2 with pi/2 split in three
-// parts, then the Cephes minimax polynomials on [-pi/4, pi/4].
-__device__ __forceinline__ float reduce_quadrant(float x, int &q) {
-    float fq = rintf(x * 0.636619772367581343f);
-    q = static_cast<int>(fq);
-    float r = x - fq * 1.5703125f;
-    r = r - fq * 4.837512969970703125e-4f;
-    r = r - fq * 7.54978995489188216e-8f;
-    return r;
-}
-__device__ __forceinline__ float poly_sin(float r) {
-    float z = r * r;
-    float p = -1.9515295891e-4f;
-    p = p * z + 8.3321608736e-3f;
-    p = p * z + -1.6666654611e-1f;
-    return r + r * z * p;
-}
-__device__ __forceinline__ float poly_cos(float r) {
-    float z = r * r;
-    float p = 2.443315711809948e-5f;
-    p = p * z + -1.388731625493765e-3f;
-    p = p * z + 4.166664568298827e-2f;
-    return (1.0f - 0.5f * z) + z * z * p;
-}
-__device__ __forceinline__ void sin_cos(float x, float &s, float &c) {
-    int q;
-    float r = reduce_quadrant(x, q);
-    float ps = poly_sin(r), pc = poly_cos(r);
-    switch (q & 3) {
-    case 0: s = ps; c = pc; break;
-    case 1: s = pc; c = -ps; break;
-    case 2: s = -ps; c = -pc; break;
-    default: s = -pc; c = ps; break;
-    }
-}
-
-// PCG4D, common.wgsl:27-41
-__device__ __forceinline__ float4 hash4f(uint32_t a, uint32_t b, uint32_t c, uint32_t d) {
-    uint32_t x = a * 1664525u + 1013904223u, y = b * 1664525u + 1013904223u;
-    uint32_t z = c * 1664525u + 1013904223u, w = d * 1664525u + 1013904223u;
-    x += y * w; y += z * x; z += x * y; w += y * z;
-    x ^= x >> 16; y ^= y >> 16; z ^= z >> 16; w ^= w >> 16;
-    x += y * w; y += z * x; z += x * y; w += y * z;
-    return make_float4(__uint_as_float((x >> 9) | 0x3F800000u) - 1.0f, __uint_as_float((y >> 9) | 0x3F800000u) - 1.0f,
-                       __uint_as_float((z >> 9) | 0x3F800000u) - 1.0f, __uint_as_float((w >> 9) | 0x3F800000u) - 1.0f);
-}
-
-struct Ray { F3 o, d, inv; };
-
-// Slab test, raytracing_sw.wgsl:81-89 (NaN-ignoring min / max: v_min_f32 / v_max_f32).
-__device__ __forceinline__ float slab(const Ray &r, float lx, float ly, float lz, float hx, float hy, float hz) {
-    float ax = (lx - r.o.x) * r.inv.x, ay = (ly - r.o.y) * r.inv.y, az = (lz - r.o.z) * r.inv.z;
-    float bx = (hx - r.o.x) * r.inv.x, by = (hy - r.o.y) * r.inv.y, bz = (hz - r.o.z) * r.inv.z;
-    float tn = fmaxf(fminf(ax, bx), fmaxf(fminf(ay, by), fminf(az, bz)));
-    float tf = fminf(fmaxf(ax, bx), fminf(fmaxf(ay, by), fmaxf(az, bz)));
-    return (tn <= tf && tf >= 0.0f) ? tn : kNoHit;
-}
-
-struct LaneCounters { uint32_t rays, aabb, tri, inst, hits; };
-
-struct Hit { float t, u, v; uint32_t tri, inst; };
-
-// Traversal stack: top K entries per lane in LDS, older entries in private memory.
-template <int K>
-struct Stack {
-    uint32_t *ref;  // LDS, slot s of this lane at ref[s * 64]
-    float *dist;
-    uint32_t sp, lo;
-    uint2 spill[kStackEntries];
-
-    __device__ __forceinline__ void push(uint32_t r, float t) {
-        if (sp >= kStackEntries) return;  // unreachable for trees of depth <= 32 (checked at upload)
-        if (sp - lo == static_cast<uint32_t>(K)) {
-            uint32_t s = lo & (K - 1);
-            spill[lo] = make_uint2(ref[s * 64], __float_as_uint(dist[s * 64]));
-            ++lo;
-        }
-        uint32_t s = sp & (K - 1);
-        ref[s * 64] = r;
-        dist[s * 64] = t;
-        ++sp;
-    }
-    __device__ __forceinline__ void pop(uint32_t &r, float &t) {
-        --sp;
-        if (sp < lo) {
-            uint2 e = spill[sp];
-            lo = sp;
-            r = e.x;
-            t = __uint_as_float(e.y);
-        } else {
-            uint32_t s = sp & (K - 1);
-            r = ref[s * 64];
-            t = dist[s * 64];
-        }
-    }
-};
-
-__device__ __forceinline__ uint32_t leaf_count(uint32_t ref, const uint32_t *side) {
-    uint32_t c = (ref >> kLeafStartBits) & kLeafMaxInline;
-    return c != 0 ? c : side[ref & kLeafStartMask];
-}
-
-// Closest hit of `world` over the two-level BVH (intersect_TLAS / intersect_BLAS,
-// raytracing_sw.wgsl:176-319).
-template <int K>
-__device__ void trace(const DevScene &S, const Ray &world, Hit &hit, Stack<K> &st, LaneCounters &c) {
-    hit.t = kNoHit;
-    hit.tri = 0; hit.inst = 0; hit.u = 0.0f; hit.v = 0.0f;
-    if (S.tlas_root == kNoRecord) return;
-    Ray ray = world;
-    st.sp = 0; st.lo = 0;
-    uint32_t blas_base = kNoRecord, inst_cur = 0, inst_end = 0;
-    uint32_t cur = S.tlas_root;
-    bool have = true;
-    for (;;) {
-        if (have) {
-            if (!(cur & kLeafBit)) {
-                const float4 *rec = S.pairs + 4 * static_cast<size_t>(cur);
-                float4 q0 = rec[0], q1 = rec[1], q2 = rec[2];
-                uint4 q3 = *reinterpret_cast<const uint4 *>(rec + 3);
-                float tl = slab(ray, q0.x, q0.y, q0.z, q0.w, q1.x, q1.y);
-                float tr = slab(ray, q1.z, q1.w, q2.x, q2.y, q2.z, q2.w);
-                c.aabb += q3.z ? 1u : 2u;
-                // ties: the right child is the near one (raytracing_sw.wgsl:229-234)
-                bool left_near = tl < tr;
-                uint32_t nref = left_near ? q3.x : q3.y, fref = left_near ? q3.y : q3.x;
-                float nt = left_near ? tl : tr, ft = left_near ? tr : tl;
-                if (ft < hit.t) st.push(fref, ft);
-                if (nt < hit.t) { cur = nref; continue; }
-                have = false;
-            } else if (blas_base == kNoRecord) {
-                // TLAS leaf: instances [start, start + count), each searched in order
-                inst_cur = cur & kLeafStartMask;
-                inst_end = inst_cur + leaf_count(cur, S.inst_leaf_count);
-                goto enter_instance;
-            } else {
-                const uint32_t start = cur & kLeafStartMask, end = start + leaf_count(cur, S.tri_leaf_count);
-                for (uint32_t j = start; j < end; ++j) {
-                    const float4 *tr = S.tris + 3 * static_cast<size_t>(j);
-                    float4 a = tr[0], b = tr[1], e = tr[2];
-                    ++c.tri;
-                    // Möller–Trumbore with back-face culling, raytracing_sw.wgsl:51-78
-                    F3 e1 = xyz(b), e2 = xyz(e);
-                    F3 h = cross(ray.d, e2);
-                    float det = dot(e1, h);
-                    if (det < kEps) continue;
-                    float inv_det = 1.0f / det;
-                    F3 s = sub(ray.o, xyz(a));
-                    float u = inv_det * dot(s, h);
-                    if (u < 0.0f || u > 1.0f) continue;
-                    F3 q = cross(s, e1);
-                    float v = inv_det * dot(ray.d, q);
-                    if (v < 0.0f || u + v > 1.0f) continue;
-                    float t = inv_det * dot(e2, q);
-                    if (t < kBias) continue;
-                    if (t < hit.t) { hit.t = t; hit.u = u; hit.v = v; hit.tri = j; hit.inst = inst_cur; }
-                }
-                have = false;
-            }
-        }
-        // pop the next entry
-        if (blas_base != kNoRecord && st.sp == blas_base) {
-            if (++inst_cur < inst_end) goto enter_instance;
-            blas_base = kNoRecord;
-            ray = world;
-        }
-        if (st.sp == 0) break;
-        {
-            float t;
-            st.pop(cur, t);
-            have = t < hit.t;   // skip entries farther than the current hit
-        }
-        continue;
-    enter_instance: {
-            const float4 *rec = S.inst + 4 * static_cast<size_t>(inst_cur);
-            float4 r0 = rec[0], r1 = rec[1], r2 = rec[2];
-            uint32_t root = __float_as_uint(rec[3].x);
-            // (W2L * (o, 1)).xyz and mat3(W2L) * d, raytracing_sw.wgsl:203-205
-            ray.o = f3(((r0.x * world.o.x + r0.y * world.o.y) + r0.z * world.o.z) + r0.w * 1.0f,
-                       ((r1.x * world.o.x + r1.y * world.o.y) + r1.z * world.o.z) + r1.w * 1.0f,
-                       ((r2.x * world.o.x + r2.y * world.o.y) + r2.z * world.o.z) + r2.w * 1.0f);
-            ray.d = f3((r0.x * world.d.x + r0.y * world.d.y) + r0.z * world.d.z,
-                       (r1.x * world.d.x + r1.y * world.d.y) + r1.z * world.d.z,
-                       (r2.x * world.d.x + r2.y * world.d.y) + r2.z * world.d.z);
-            ray.inv = rcp3(ray.d);
-            ++c.inst;
-            blas_base = st.sp;
-            cur = root;
-            have = true;
-        }
-    }
-}
-
-// Bilinear cube lookup at LOD 0 with Vulkan face selection; faces carry a one-texel border
-// holding the seamless-edge neighbours, so the four taps never leave the padded face.
-__device__ F3 env_sample(const DevEnv &E, F3 d) {
-    float ax = fabsf(d.x), ay = fabsf(d.y), az = fabsf(d.z);
-    int f;
-    float ma, sc, tc;
-    if (ax >= ay && ax >= az) {
-        f = d.x >= 0.0f ? 0 : 1; ma = ax;
-        sc = f == 0 ? -1.0f * d.z : 1.0f * d.z; tc = -1.0f * d.y;
-    } else if (ay >= az) {
-        f = d.y >= 0.0f ? 2 : 3; ma = ay;
-        sc = 1.0f * d.x; tc = f == 2 ? 1.0f * d.z : -1.0f * d.z;
-    } else {
-        f = d.z >= 0.0f ? 4 : 5; ma = az;
-        sc = f == 4 ? 1.0f * d.x : -1.0f * d.x; tc = -1.0f * d.y;
-    }
-    float s = 0.5f * (sc / ma + 1.0f), t = 0.5f * (tc / ma + 1.0f);
-    float n = static_cast<float>(E.n);
-    float u = s * n - 0.5f, v = t * n - 0.5f;
-    float fu = floorf(u), fv = floorf(v);
-    if (!(fu >= -1.0f)) fu = -1.0f;
-    if (fu > n - 1.0f) fu = n - 1.0f;
-    if (!(fv >= -1.0f)) fv = -1.0f;
-    if (fv > n - 1.0f) fv = n - 1.0f;
-    float a = u - fu, b = v - fv;
-    const uint32_t pn = E.n + 2;
-    const int i0 = static_cast<int>(fu) + 1, j0 = static_cast<int>(fv) + 1;
-    const float4 *base = E.texels + (static_cast<size_t>(f) * pn + j0) * pn + i0;
-    float4 t00 = base[0], t10 = base[1], t01 = base[pn], t11 = base[pn + 1];
-    float w00 = (1.0f - a) * (1.0f - b), w10 = a * (1.0f - b), w01 = (1.0f - a) * b, w11 = a * b;
-    return f3(((w00 * t00.x + w10 * t10.x) + w01 * t01.x) + w11 * t11.x,
-              ((w00 * t00.y + w10 * t10.y) + w01 * t01.y) + w11 * t11.y,
-              ((w00 * t00.z + w10 * t10.z) + w01 * t01.z) + w11 * t11.z);
-}
-
-__device__ __forceinline__ float4 sobol_fetch(const RenderParams &p, uint32_t entry, const float4 &shift) {
-    float4 s = entry < p.sobol_entries ? p.sobol[entry] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    return make_float4(fract(s.x + shift.x), fract(s.y + shift.y), fract(s.z + shift.z), fract(s.w + shift.w));
-}
-
-// generate_ray, pathtracing.wgsl:27-41
-__device__ __forceinline__ Ray camera_ray(const RenderParams &p, uint32_t x, uint32_t y, float jx, float jy) {
-    const float *m = p.c2w;
-    float uvx = (2.0f * (static_cast<float>(x) + jx)) / static_cast<float>(p.width) - 1.0f;
-    float uvy = (2.0f * (static_cast<float>(y) + jy)) / static_cast<float>(p.height) - 1.0f;
-    F3 pos = f3(m[12] / m[15], m[13] / m[15], m[14] / m[15]);
-    float cx = -uvx, cy = -uvy;
-    float wx = ((m[0] * cx + m[4] * cy) + m[8] * -1.0f) + m[12] * 1.0f;
-    float wy = ((m[1] * cx + m[5] * cy) + m[9] * -1.0f) + m[13] * 1.0f;
-    float wz = ((m[2] * cx + m[6] * cy) + m[10] * -1.0f) + m[14] * 1.0f;
-    float ww = ((m[3] * cx + m[7] * cy) + m[11] * -1.0f) + m[15] * 1.0f;
-    Ray r;
-    r.o = pos;
-    r.d = sub(pos, f3(wx / ww, wy / ww, wz / ww));
-    r.inv = rcp3(r.d);
-    return r;
-}
-
-// sample_vndf_iso, pathtracing.wgsl:68-98
-__device__ F3 sample_vndf_iso(float r0, float r1, F3 wi, float alpha, F3 n) {
-    if (alpha == 0.0f) return n;
-    F3 wi_z = scale(dot(wi, n), n);
-    F3 wi_xy = sub(wi, wi_z);
-    F3 wi_std = normalize(sub(wi_z, scale(alpha, wi_xy)));
-    float wi_std_z = dot(wi_std, n);
-    float phi = (2.0f * r0 - 1.0f) * kPi;
-    float z = (1.0f - r1) * (1.0f + wi_std_z) - wi_std_z;
-    float sin_theta = sqrtf(fminf(fmaxf(1.0f - z * z, 0.0f), 1.0f));
-    float sp, cp;
-    sin_cos(phi, sp, cp);
-    F3 c_std = f3(sin_theta * cp, sin_theta * sp, z);
-    F3 wr = add(n, f3(0.0f, 0.0f, 1.0f));
-    float safe_wrz = fmaxf(wr.z, 1e-6f);
-    float dwc = dot(wr, c_std);
-    F3 c = sub(divs(scale(dwc, wr), safe_wrz), c_std);
-    F3 wm_std = add(c, wi_std);
-    F3 wm_std_z = scale(dot(n, wm_std), n);
-    F3 wm_std_xy = sub(wm_std_z, wm_std);
-    return normalize(add(wm_std_z, scale(alpha, wm_std_xy)));
-}
-
-__device__ __forceinline__ F3 schlick(float hdotv, F3 r0) {
-    float p = pow5(1.0f - hdotv);
-    return f3(r0.x + (1.0f - r0.x) * p, r0.y + (1.0f - r0.y) * p, r0.z + (1.0f - r0.z) * p);
-}
-
-__device__ __forceinline__ float lambda_tr(float ndotv, float alpha2) {
-    float c2 = ndotv * ndotv;
-    float s2 = 1.0f - c2;
-    float t2 = s2 / c2;
-    return (-1.0f + sqrtf(1.0f + alpha2 * t2)) / 2.0f;
-}
-
-// One path: sample_rendering_eq (pathtracing.wgsl:161-241) with intersect_scene
-// (raytracing_sw.wgsl:138-173) inlined.
-template <int K>
-__device__ F3 sample_path(const RenderParams &p, uint32_t sample, const float4 &shift, Ray ray, Stack<K> &st,
-                          LaneCounters &c) {
-    const DevScene &S = p.scene;
-    F3 thr = f3(1.0f, 1.0f, 1.0f);
-    for (uint32_t bounce = 0; bounce <= p.bounces; ++bounce) {
-        Hit h;
-        trace<K>(S, ray, h, st, c);
-        ++c.rays;
-        if (h.t == kNoHit) return mul(thr, env_sample(p.env, ray.d));
-        ++c.hits;
-        const float4 *sh = S.ishade + 6 * static_cast<size_t>(h.inst);
-        float4 s0 = sh[0], s1 = sh[1], s2 = sh[2], s3 = sh[3], s4 = sh[4], s5 = sh[5];
-        F3 albedo = f3(s3.w, s4.w, s5.w);
-        if (s2.w > 0.0f) return mul(thr, albedo);
-        const float4 *tr = S.tris + 3 * static_cast<size_t>(h.tri);
-        uint32_t i0 = __float_as_uint(tr[0].w), i1 = __float_as_uint(tr[1].w), i2 = __float_as_uint(tr[2].w);
-        float4 n0 = S.vshade[2 * i0], t0 = S.vshade[2 * i0 + 1];
-        float4 n1 = S.vshade[2 * i1], t1 = S.vshade[2 * i1 + 1];
-        float4 n2 = S.vshade[2 * i2], t2 = S.vshade[2 * i2 + 1];
-        float bx = 1.0f - h.u - h.v, by = h.u, bz = h.v;
-        F3 pos = add(ray.o, scale(h.t, ray.d));
-        F3 ln = add(add(scale(bx, xyz(n0)), scale(by, xyz(n1))), scale(bz, xyz(n2)));
-        F3 lt = f3((t0.x * bx + t1.x * by) + t2.x * bz, (t0.y * bx + t1.y * by) + t2.y * bz,
-                   (t0.z * bx + t1.z * by) + t2.z * bz);
-        F3 nrm = f3((s0.x * ln.x + s0.y * ln.y) + s0.z * ln.z, (s1.x * ln.x + s1.y * ln.y) + s1.z * ln.z,
-                    (s2.x * ln.x + s2.y * ln.y) + s2.z * ln.z);
-        F3 tan = f3((s3.x * lt.x + s4.x * lt.y) + s5.x * lt.z, (s3.y * lt.x + s4.y * lt.y) + s5.y * lt.z,
-                    (s3.z * lt.x + s4.z * lt.y) + s5.z * lt.z);
-        float roughness = s0.w, metallic = s1.w;
-
-        float alpha = roughness * roughness;
-        float alpha2 = alpha * alpha;
-        F3 n = normalize(nrm);
-        uint32_t e0 = sample * p.sobol_stride + 1u + bounce * 2u;
-        float4 sob0 = sobol_fetch(p, e0, shift), sob1 = sobol_fetch(p, e0 + 1u, shift);
-        F3 wo = normalize(f3(-ray.d.x, -ray.d.y, -ray.d.z));
-        float cos_o = dot(wo, n);
-        F3 wi;
-        F3 f0 = f3(0.04f * (1.0f - metallic) + albedo.x * metallic, 0.04f * (1.0f - metallic) + albedo.y * metallic,
-                   0.04f * (1.0f - metallic) + albedo.z * metallic);
-        float spec_w = luminance(schlick(dot(wo, n), f0));
-        float diff_w = (1.0f - metallic) * luminance(albedo);
-        float p_spec = spec_w / (spec_w + diff_w);
-        float p_diff = 1.0f - p_spec;
-        if (sob0.x < p_spec) {
-            F3 wm = sample_vndf_iso(sob0.y, sob0.z, wo, alpha, n);
-            F3 mwo = f3(-wo.x, -wo.y, -wo.z);
-            float k2 = 2.0f * dot(wm, mwo);
-            wi = sub(mwo, scale(k2, wm));
-            float cos_d = dot(wo, wm);
-            float cos_i = dot(wi, n);
-            F3 F = schlick(cos_d, f0);
-            float ll = lambda_tr(cos_i, alpha2), lv = lambda_tr(cos_o, alpha2);
-            float den = (1.0f + ll) + lv;
-            F3 spec = f3((F.x * (1.0f + lv)) / den, (F.y * (1.0f + lv)) / den, (F.z * (1.0f + lv)) / den);
-            thr = mul(thr, divs(spec, p_spec));
-        } else {
-            F3 nt = normalize(sub(tan, scale(dot(tan, n), n)));
-            F3 b = cross(n, nt);
-            float phi = kTwoPi * sob1.y;
-            float st_ = sqrtf(1.0f - sob1.z), ct = sqrtf(sob1.z);
-            float sp, cp;
-            sin_cos(phi, sp, cp);
-            F3 cs = f3(cp * st_, sp * st_, ct);
-            wi = add(add(scale(cs.x, nt), scale(cs.y, b)), scale(cs.z, n));
-            F3 wm = normalize(add(wi, wo));
-            float cos_d = dot(wi, wm);
-            float cos_i = dot(wi, n);
-            float fd90 = 0.5f + (2.0f * alpha) * (cos_d * cos_d);
-            float resp = (1.0f + (fd90 - 1.0f) * pow5(1.0f - cos_i)) * (1.0f + (fd90 - 1.0f) * pow5(1.0f - cos_o));
-            float om = 1.0f - metallic;
-            F3 diff = f3((om * albedo.x) * resp, (om * albedo.y) * resp, (om * albedo.z) * resp);
-            thr = mul(thr, divs(diff, p_diff));
-        }
-        // Russian roulette, pathtracing.wgsl:227-236
-        float p_cont = fminf(1.0f - pow8(static_cast<float>(bounce) / static_cast<float>(p.bounces)), 1.0f);
-        p_cont *= fminf(luminance(thr) * p.contribution_factor, 1.0f);
-        if (sob0.z < p_cont) thr = divs(thr, p_cont);
-        else return f3(0.0f, 0.0f, 0.0f);
-        ray.o = pos;
-        ray.d = wi;
-        ray.inv = rcp3(wi);
-    }
-    return f3(0.0f, 0.0f, 0.0f);
-}
+using namespace dev;
 
 __device__ __forceinline__ uint32_t wave_sum(uint32_t v) {
     for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
@@ -462,15 +60,22 @@ __global__ __launch_bounds__(64) void render_kernel(RenderParams p) {
     st.dist = s_dist + lane;
     LaneCounters c{0, 0, 0, 0, 0};
     float4 *px = p.target + static_cast<size_t>(y) * p.width + x;
-    const float4 shift = hash4f(x, y, x, y);
+    PathConst pc;
+    pc.scene = p.scene;
+    pc.env = p.env;
+    pc.sobol = SobolView{p.sobol, p.sobol_entries, p.sobol_stride};
+    pc.bounces = p.bounces;
+    pc.contribution_factor = p.contribution_factor;
+    pc.shift = hash4f(x, y, x, y);
     F3 col = f3(0.0f, 0.0f, 0.0f);
     for (uint32_t f = 0; f < p.n_frames; ++f) {
         const uint32_t sample = p.first_sample + f;
         const float w = (f == 0 && !p.weights_from_sample) ? p.first_weight : 1.0f / static_cast<float>(sample);
         if (f == 0 && w > 0.0f) { float4 prev = *px; col = f3(prev.x, prev.y, prev.z); }
-        float4 jit = sobol_fetch(p, sample * p.sobol_stride, shift);
-        Ray ray = camera_ray(p, x, y, jit.x, jit.y);
-        F3 s = sample_path<K>(p, sample, shift, ray, st, c);
+        pc.sample = sample;
+        float4 jit = sobol_fetch(pc.sobol, sample * p.sobol_stride, pc.shift);
+        Ray ray = camera_ray(p.c2w, p.width, p.height, x, y, jit.x, jit.y);
+        F3 s = sample_path<K>(pc, ray, st, c);
         // mix(color, s, weight) = color * (1 - weight) + s * weight
         col = f3(col.x * (1.0f - w) + s.x * w, col.y * (1.0f - w) + s.y * w, col.z * (1.0f - w) + s.z * w);
     }
@@ -491,8 +96,8 @@ __global__ __launch_bounds__(64) void primary_hits_kernel(RenderParams p) {
     st.dist = s_dist + lane;
     LaneCounters c{0, 0, 0, 0, 0};
     const float4 shift = hash4f(x, y, x, y);
-    float4 jit = sobol_fetch(p, p.first_sample * p.sobol_stride, shift);
-    Ray ray = camera_ray(p, x, y, jit.x, jit.y);
+    float4 jit = sobol_fetch(SobolView{p.sobol, p.sobol_entries, p.sobol_stride}, p.first_sample * p.sobol_stride, shift);
+    Ray ray = camera_ray(p.c2w, p.width, p.height, x, y, jit.x, jit.y);
     Hit h;
     trace<K>(p.scene, ray, h, st, c);
     uint32_t *o = p.hits_out + 5 * (static_cast<size_t>(y) * p.width + x);

@@ -1,0 +1,157 @@
+"""GPU tests of the serving path (nb_server_*): each request's image equals the plain
+renderer's and does not depend on the other requests in its steps; admission follows the HBM
+budget; requests that arrive while others run join the running steps."""
+import numpy as np
+import pytest
+
+from tests import _workloads as W
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def env(nb_mod=None):
+    import nbounce_amd as nb
+    from nbounce_amd import scenegen
+    ctx = nb.Context(0)
+    t = W.table()
+    import ctypes
+    from nbounce_amd._lib import lib, ptr, check
+    check(lib.nb_sobol_set(ctx.handle, ptr(np.ascontiguousarray(t)), t.shape[0], t.shape[1], 17))
+    small = W.small()
+    sb_small = nb.SceneBuffers(ctx, small["blas"], small["tlas"], small["instances"], small["vertices"], small["indices"])
+    w = W.large(0, 512)
+    sc = nb.Scene(w["vertices"], w["indices_in"].copy(), w["primitives"])
+    b, tl, inst = nb.build_scene(sc, share_blas=True)
+    sb_large = nb.SceneBuffers(ctx, b, tl, inst, sc.vertices, sc.indices)
+    e = nb.EnvMap(ctx, scenegen.sky_cube(32))
+    yield nb, ctx, [sb_small, sb_large], e
+    ctx.close()
+
+
+def _cam(nb, eye, target, w, h):
+    return nb.CameraController(world_position=eye, target=target, aspect_ratio=w / h).data()
+
+
+def _plain(nb, ctx, sb, e, cam, w, h, spp, bounces):
+    from nbounce_amd._lib import lib, ptr, check
+    import ctypes
+    t = ctypes.c_void_p()
+    check(lib.nb_target_create(ctx.handle, w, h, ctypes.byref(t)))
+    g = nb.Globals(1, 0.0, bounces, 4.0).array()
+    check(lib.nb_render_frames(ctx.handle, sb.handle, t, ptr(cam), ptr(g), spp, e.handle))
+    img = np.zeros((h, w, 4), dtype=np.float32)
+    check(lib.nb_target_read(t, ptr(img)))
+    lib.nb_target_destroy(t)
+    return img
+
+
+def _run_to_completion(srv, max_steps=1000):
+    done = {}
+    for _ in range(max_steps):
+        a, q = srv.step()
+        for c in srv.poll():
+            done[int(c["id"])] = c
+        if a == 0 and q == 0:
+            break
+    return done
+
+
+REQ = dict(scene=1, eye=(-20.0, 110.0, -20.0), target=(100.0, 95.0, 100.0), w=96, h=64, spp=5, bounces=8)
+
+
+def test_request_equals_plain_render(env):
+    from nbounce_amd.serving import Server
+    nb, ctx, scenes, e = env
+    print("case: one served request vs nb_render_frames", flush=True)
+    srv = Server(ctx, max_paths=1 << 16, max_bounces=12)
+    for s in scenes:
+        srv.add_scene(s, e)
+    cam = _cam(nb, REQ["eye"], REQ["target"], REQ["w"], REQ["h"])
+    rid = srv.submit(cam, REQ["scene"], REQ["w"], REQ["h"], REQ["spp"], REQ["bounces"])
+    done = _run_to_completion(srv)
+    assert rid in done
+    got = srv.read(rid, REQ["w"], REQ["h"])
+    want = _plain(nb, ctx, scenes[1], e, cam, REQ["w"], REQ["h"], REQ["spp"], REQ["bounces"])
+    np.testing.assert_array_equal(got.view(np.uint32), want.view(np.uint32))
+    st = srv.stats()
+    assert st["samples"] == REQ["w"] * REQ["h"] * REQ["spp"]
+    srv.close()
+
+
+def test_request_is_the_same_alone_and_in_a_mix(env):
+    from nbounce_amd.serving import Server
+    nb, ctx, scenes, e = env
+    print("case: request alone vs in a mix of 8 requests over 2 scenes", flush=True)
+    cam = _cam(nb, REQ["eye"], REQ["target"], REQ["w"], REQ["h"])
+    alone = Server(ctx, max_paths=1 << 16, max_bounces=12)
+    for s in scenes:
+        alone.add_scene(s, e)
+    rid = alone.submit(cam, REQ["scene"], REQ["w"], REQ["h"], REQ["spp"], REQ["bounces"])
+    _run_to_completion(alone)
+    img_alone = alone.read(rid, REQ["w"], REQ["h"])
+    alone.close()
+
+    mix = Server(ctx, max_paths=1 << 15, max_bounces=12)
+    for s in scenes:
+        mix.add_scene(s, e)
+    rng = np.random.default_rng(3)
+    others = []
+    for k in range(7):
+        w, h = [(64, 64), (128, 96), (32, 32)][k % 3]
+        sc = int(k % 2)
+        eye = (5.0, 5.0, 9.5) if sc == 0 else tuple(rng.uniform(0, 200, 3))
+        tgt = (5.0, 4.0, 0.0) if sc == 0 else tuple(rng.uniform(0, 200, 3))
+        others.append(mix.submit(_cam(nb, eye, tgt, w, h), sc, w, h, int(rng.integers(1, 9)), int(rng.integers(1, 13))))
+        if k == 3:
+            rid2 = mix.submit(cam, REQ["scene"], REQ["w"], REQ["h"], REQ["spp"], REQ["bounces"])
+            mix.step()     # the request arrives while earlier ones are being rendered
+    done = _run_to_completion(mix)
+    assert rid2 in done and all(o in done for o in others)
+    img_mix = mix.read(rid2, REQ["w"], REQ["h"])
+    assert mix.stats()["steps"] > 2
+    mix.close()
+    np.testing.assert_array_equal(img_alone.view(np.uint32), img_mix.view(np.uint32))
+
+
+def test_admission_waits_for_hbm_budget(env):
+    from nbounce_amd.serving import Server
+    nb, ctx, scenes, e = env
+    print("case: admission under a budget of 1.5 images", flush=True)
+    w = h = 64
+    srv = Server(ctx, hbm_budget_bytes=int(1.5 * w * h * 16), max_paths=1 << 16, max_bounces=8)
+    srv.add_scene(scenes[0], e)
+    cam = _cam(nb, (5.0, 5.0, 9.5), (5.0, 4.0, 0.0), w, h)
+    ids = [srv.submit(cam, 0, w, h, 1, 8) for _ in range(3)]
+    a, q = srv.step()
+    assert (a, q) == (0, 2)                     # first admitted and finished, two wait
+    first = srv.poll()
+    assert [int(c["id"]) for c in first] == ids[:1]
+    a, q = srv.step()
+    assert (a, q) == (0, 2) and len(srv.poll()) == 0   # its image still holds the budget
+    srv.release(ids[0])
+    srv.step()
+    assert [int(c["id"]) for c in srv.poll()] == ids[1:2]
+    with pytest.raises(nb.NbError):
+        srv.submit(_cam(nb, (5, 5, 9.5), (5, 4, 0), 128, 128), 0, 128, 128, 1, 8)   # larger than the budget
+    srv.close()
+
+
+def test_small_requests_overtake_a_long_one(env):
+    from nbounce_amd.serving import Server
+    nb, ctx, scenes, e = env
+    print("case: continuous batching, small requests join a long one", flush=True)
+    srv = Server(ctx, max_paths=1 << 17, max_bounces=8)
+    srv.add_scene(scenes[0], e)
+    big = srv.submit(_cam(nb, (5.0, 5.0, 9.5), (5.0, 4.0, 0.0), 256, 256), 0, 256, 256, 32, 8)
+    srv.step()
+    small = [srv.submit(_cam(nb, (5.0, 5.0, 9.5), (5.0, 4.0, 0.0), 32, 32), 0, 32, 32, 2, 8) for _ in range(3)]
+    order = []
+    for _ in range(200):
+        a, q = srv.step()
+        order += [int(c["id"]) for c in srv.poll()]
+        if a == 0 and q == 0:
+            break
+    assert set(order) == set(small) | {big}
+    assert order.index(big) == len(order) - 1
+    srv.close()
