@@ -229,6 +229,15 @@ def main():
     bytes_per_launch = algorithmic_bytes(c) / launches
     achieved = bytes_per_launch / (kernel_ms * 1e-3) / 1e9
 
+    traffic = None
+    try:
+        with open(os.path.join(ROOT, "profiles", "traffic.json")) as f:
+            key = "render_kernel instanced 1920x1080 12 bounces shared BLAS"
+            if args.workload == "large" and not args.reference_blas and F == 1:
+                traffic = json.load(f).get(key, {}).get("hbm_bytes_per_launch")
+    except OSError:
+        pass
+
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(P, scene, arrays, env_faces, pt._camera_data, pt.table, target_s=args.cpu_seconds)
@@ -263,7 +272,9 @@ def main():
             "peak": PEAK_HBM_GBS,
             "unit": "GB/s",
             "frac": round(achieved / PEAK_HBM_GBS, 5),
-            "traffic": None,
+            "traffic": traffic,
+            "traffic_source": "rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE, scaled by a 256 MiB copy "
+                              "(profiles/render_kernel_instanced_1080p_traffic.md)" if traffic else None,
             "kernel": "render_kernel",
             "kernel_ms": round(kernel_ms, 4),
             "algorithmic_bytes_per_launch": int(bytes_per_launch),
