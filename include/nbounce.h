@@ -218,6 +218,30 @@ nb_status nb_server_release(nb_server *srv, uint64_t id);
 nb_status nb_server_now(nb_server *srv, double *seconds);
 nb_status nb_server_stats_get(nb_server *srv, nb_server_stats *out);
 
+/* ---- learned radiance cache ------------------------------------------------------------------
+ * No counterpart in the reference (its README plans Neural Radiance Caching, README.md:12,23).
+ * An MLP (96 encoded inputs -> 5 x 64 ReLU -> 3) maps a shading point to outgoing radiance; BF16
+ * weights and activations on the matrix cores, FP32 accumulation and FP32 master parameters,
+ * Adam on a relative L2 loss.  Raw records are 16 floats: position xyz, outgoing direction xyz,
+ * normal xyz, roughness, albedo rgb, metallic, 0, 0.  Parameters are one flat fp32 vector:
+ * W0 (64x96), b0, W1..W4 (64x64) with biases, W5 (3x64), b5; W is (out x in) row-major. */
+typedef struct nb_cache nb_cache;
+
+uint32_t nb_cache_param_count(void);
+nb_status nb_cache_create(nb_context *ctx, const float scene_min[3], const float scene_max[3], uint32_t seed,
+                          uint32_t max_batch, nb_cache **out);
+nb_status nb_cache_destroy(nb_cache *cache);
+nb_status nb_cache_get_params(nb_cache *cache, float *out);
+nb_status nb_cache_set_params(nb_cache *cache, const float *in);
+/* Host arrays in and out (used by tests and offline tools): the bf16 input encoding, the
+ * network output, the loss and its gradient (no update), and Adam steps over minibatches. */
+nb_status nb_cache_encode(nb_cache *cache, const float *raw, uint32_t n, uint16_t *encoded);
+nb_status nb_cache_infer(nb_cache *cache, const float *raw, uint32_t n, float *rgb);
+nb_status nb_cache_gradient(nb_cache *cache, const float *raw, const float *target, uint32_t n, float *loss,
+                            float *grad);
+nb_status nb_cache_train(nb_cache *cache, const float *raw, const float *target, uint32_t n, uint32_t batch,
+                         float lr, float *last_loss);
+
 #ifdef __cplusplus
 }
 #endif

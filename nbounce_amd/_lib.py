@@ -56,6 +56,14 @@ SIGNATURES = {
     "nb_server_release": [_p, _c.c_uint64],
     "nb_server_now": [_p, _ptr(_c.c_double)],
     "nb_server_stats_get": [_p, _p],
+    "nb_cache_create": [_p, _p, _p, _u32, _u32, _c.POINTER(_p)],
+    "nb_cache_destroy": [_p],
+    "nb_cache_get_params": [_p, _p],
+    "nb_cache_set_params": [_p, _p],
+    "nb_cache_encode": [_p, _p, _u32, _p],
+    "nb_cache_infer": [_p, _p, _u32, _p],
+    "nb_cache_gradient": [_p, _p, _p, _u32, _ptr(_f32), _p],
+    "nb_cache_train": [_p, _p, _p, _u32, _u32, _f32, _ptr(_f32)],
 }
 
 
@@ -79,6 +87,8 @@ def _load():
     lib.nb_last_error.restype = _c.c_char_p
     lib.nb_version.argtypes = []
     lib.nb_version.restype = _u32
+    lib.nb_cache_param_count.argtypes = []
+    lib.nb_cache_param_count.restype = _u32
     return lib
 
 

@@ -1,0 +1,273 @@
+// Learned radiance cache: parameters, Adam state, training and inference buffers of the MLP in
+// mlp.hip, and the nb_cache_* entry points.  The MLP maps a shading point (position, outgoing
+// direction, normal, roughness, albedo, metallic) to outgoing radiance; it is trained online with
+// Adam on a relative L2 loss against radiance estimates from the tracer's own path suffixes.
+#include "nbounce.h"
+#include "nb_internal.h"
+
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+using namespace nb;
+
+namespace {
+
+#define C_HIP(call)                                                                                    \
+    do {                                                                                               \
+        hipError_t e_ = (call);                                                                        \
+        if (e_ != hipSuccess)                                                                          \
+            return set_error(e_ == hipErrorOutOfMemory ? NB_ERR_OUT_OF_MEMORY : NB_ERR_HIP, "%s: %s", \
+                             #call, hipGetErrorString(e_));                                            \
+    } while (0)
+#define C_TRY(expr)                    \
+    do {                               \
+        nb_status s_ = (expr);         \
+        if (s_ != NB_OK) return s_;    \
+    } while (0)
+
+// Offsets of each layer's weights and biases in the flat fp32 parameter vector, and of each
+// layer's weights in the bf16 copy (the last layer padded to 16 rows).
+struct Layout {
+    uint32_t w[kMlpLayers], b[kMlpLayers], wb[kMlpLayers], in[kMlpLayers], out[kMlpLayers];
+    Layout() {
+        uint32_t off = 0, offb = 0;
+        for (int L = 0; L < kMlpLayers; ++L) {
+            in[L] = L == 0 ? kMlpIn : kMlpHidden;
+            out[L] = L == kMlpLayers - 1 ? kMlpOut : kMlpHidden;
+            w[L] = off; off += in[L] * out[L];
+            b[L] = off; off += out[L];
+            wb[L] = offb; offb += in[L] * (L == kMlpLayers - 1 ? kMlpOutPad : out[L]);
+        }
+    }
+};
+const Layout kL;
+constexpr uint32_t kBf16Weights = kMlpHidden * kMlpIn + 4 * kMlpHidden * kMlpHidden + kMlpOutPad * kMlpHidden;
+
+uint64_t splitmix(uint64_t &s) {
+    uint64_t z = (s += 0x9E3779B97F4A7C15ull);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+}  // namespace
+
+struct nb_cache {
+    nb_context *ctx = nullptr;
+    hipStream_t stream = nullptr;
+    uint32_t max_batch = 0;
+    float scene_min[3], scene_inv[3];
+    float *params = nullptr, *adam_m = nullptr, *adam_v = nullptr, *grad = nullptr;
+    uint16_t *wbf = nullptr;     // bf16 weights, last layer padded
+    float *b5 = nullptr;         // last-layer bias padded to 16
+    uint32_t adam_t = 0;
+    // batch buffers
+    float *raw = nullptr, *target = nullptr, *y = nullptr, *rgb = nullptr, *loss = nullptr;
+    uint16_t *x = nullptr, *h[5] = {}, *dz[5] = {}, *dy = nullptr;
+
+    MlpDev dev() const {
+        MlpDev m;
+        for (int L = 0; L < kMlpLayers; ++L) {
+            m.w[L] = wbf + kL.wb[L];
+            m.b[L] = L == kMlpLayers - 1 ? b5 : params + kL.b[L];
+        }
+        for (int d = 0; d < 3; ++d) { m.scene_min[d] = scene_min[d]; m.scene_inv_extent[d] = scene_inv[d]; }
+        return m;
+    }
+};
+
+namespace nb {
+
+// bf16 weight copies and the padded last-layer bias from the fp32 parameters.
+nb_status cache_refresh(nb_cache *c) {
+    for (int L = 0; L < kMlpLayers; ++L)
+        C_TRY(mlp_launch_to_bf16(c->params + kL.w[L], c->wbf + kL.wb[L], kL.in[L] * kL.out[L], c->stream));
+    C_HIP(hipMemcpyAsync(c->b5, c->params + kL.b[kMlpLayers - 1], kMlpOut * sizeof(float), hipMemcpyDeviceToDevice,
+                         c->stream));
+    return NB_OK;
+}
+
+// Forward and backward over the batch in c->raw / c->target (n rows); gradients into c->grad,
+// mean loss into c->loss.  No parameter update.
+nb_status cache_forward_backward(nb_cache *c, uint32_t n) {
+    const MlpDev m = c->dev();
+    hipStream_t s = c->stream;
+    C_TRY(mlp_launch_encode(m, c->raw, n, c->x, s));
+    const uint16_t *in = c->x;
+    for (int L = 0; L < 5; ++L) {
+        C_TRY(mlp_launch_dense_fwd(in, kL.in[L], m.w[L], m.b[L], kMlpHidden, n, nullptr, c->h[L], 1, s));
+        in = c->h[L];
+    }
+    C_TRY(mlp_launch_dense_fwd(c->h[4], kMlpHidden, m.w[5], m.b[5], kMlpOut, n, c->y, nullptr, 0, s));
+    C_TRY(mlp_launch_fill(c->loss, 1, 0.0f, s));
+    C_TRY(mlp_launch_loss(c->y, c->target, n, nullptr, c->dy, c->loss, s));
+    C_TRY(mlp_launch_fill(c->grad, kMlpParams, 0.0f, s));
+    // last layer
+    C_TRY(mlp_launch_dense_bwd_weight(c->dy, kMlpOut, c->h[4], kMlpHidden, n, c->grad + kL.w[5], c->grad + kL.b[5], s));
+    C_TRY(mlp_launch_dense_bwd_data(c->dy, kMlpOutPad, m.w[5], kMlpHidden, n, c->h[4], c->dz[4], s));
+    for (int L = 4; L >= 1; --L) {
+        C_TRY(mlp_launch_dense_bwd_weight(c->dz[L], kMlpHidden, c->h[L - 1], kMlpHidden, n, c->grad + kL.w[L],
+                                          c->grad + kL.b[L], s));
+        C_TRY(mlp_launch_dense_bwd_data(c->dz[L], kMlpHidden, m.w[L], kMlpHidden, n, c->h[L - 1], c->dz[L - 1], s));
+    }
+    C_TRY(mlp_launch_dense_bwd_weight(c->dz[0], kMlpHidden, c->x, kMlpIn, n, c->grad + kL.w[0], c->grad + kL.b[0], s));
+    return NB_OK;
+}
+
+nb_status cache_adam_step(nb_cache *c, float lr) {
+    const float b1 = 0.9f, b2 = 0.99f, eps = 1e-8f;   // Adam with the usual momentum constants
+    ++c->adam_t;
+    const float c1 = static_cast<float>(1.0 / (1.0 - std::pow(0.9, c->adam_t)));
+    const float c2 = static_cast<float>(1.0 / (1.0 - std::pow(0.99, c->adam_t)));
+    C_TRY(mlp_launch_adam(c->params, c->adam_m, c->adam_v, c->grad, kMlpParams, lr, b1, b2, eps, c1, c2, c->stream));
+    return cache_refresh(c);
+}
+
+MlpDev cache_dev(const nb_cache *c) { return c->dev(); }
+hipStream_t cache_stream(const nb_cache *c) { return c->stream; }
+float *cache_raw(nb_cache *c) { return c->raw; }
+float *cache_target(nb_cache *c) { return c->target; }
+uint32_t cache_max_batch(const nb_cache *c) { return c->max_batch; }
+
+}  // namespace nb
+
+extern "C" {
+
+uint32_t nb_cache_param_count(void) { return kMlpParams; }
+
+nb_status nb_cache_create(nb_context *ctx, const float scene_min[3], const float scene_max[3], uint32_t seed,
+                          uint32_t max_batch, nb_cache **out) {
+    if (!ctx || !scene_min || !scene_max || !out) return set_error(NB_ERR_INVALID_ARGUMENT, "null argument");
+    *out = nullptr;
+    if (max_batch < 16 || max_batch > (1u << 24)) return set_error(NB_ERR_INVALID_ARGUMENT, "max_batch must be in 16..2^24");
+    for (int d = 0; d < 3; ++d)
+        if (!(scene_max[d] > scene_min[d])) return set_error(NB_ERR_INVALID_ARGUMENT, "empty scene box");
+    C_HIP(hipSetDevice(ctx->device));
+    nb_cache *c = new nb_cache();
+    c->ctx = ctx;
+    context_add_child(ctx);
+    c->max_batch = (max_batch + 15) / 16 * 16;
+    for (int d = 0; d < 3; ++d) { c->scene_min[d] = scene_min[d]; c->scene_inv[d] = 1.0f / (scene_max[d] - scene_min[d]); }
+    const size_t B = c->max_batch;
+    hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    auto alloc = [&](auto **p, size_t bytes) { if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void **>(p), bytes); };
+    alloc(&c->params, kMlpParams * 4); alloc(&c->adam_m, kMlpParams * 4); alloc(&c->adam_v, kMlpParams * 4);
+    alloc(&c->grad, kMlpParams * 4); alloc(&c->wbf, kBf16Weights * 2); alloc(&c->b5, kMlpOutPad * 4);
+    alloc(&c->raw, B * kMlpRaw * 4); alloc(&c->target, B * 3 * 4); alloc(&c->y, B * 3 * 4); alloc(&c->rgb, B * 3 * 4);
+    alloc(&c->loss, 4); alloc(&c->x, B * kMlpIn * 2); alloc(&c->dy, B * kMlpOutPad * 2);
+    for (int L = 0; L < 5; ++L) { alloc(&c->h[L], B * kMlpHidden * 2); alloc(&c->dz[L], B * kMlpHidden * 2); }
+    if (e == hipSuccess) e = hipMemset(c->wbf, 0, kBf16Weights * 2);
+    if (e == hipSuccess) e = hipMemset(c->b5, 0, kMlpOutPad * 4);
+    if (e == hipSuccess) e = hipMemset(c->adam_m, 0, kMlpParams * 4);
+    if (e == hipSuccess) e = hipMemset(c->adam_v, 0, kMlpParams * 4);
+    if (e != hipSuccess) {
+        nb_cache_destroy(c);
+        return set_error(NB_ERR_OUT_OF_MEMORY, "cache allocation: %s", hipGetErrorString(e));
+    }
+    // He-uniform weights for the ReLU layers, a 10x smaller last layer, zero biases.
+    std::vector<float> p(kMlpParams, 0.0f);
+    uint64_t st = 0x5EEDull ^ (static_cast<uint64_t>(seed) << 32);
+    for (int L = 0; L < kMlpLayers; ++L) {
+        const float bound = std::sqrt(6.0f / static_cast<float>(kL.in[L])) * (L == kMlpLayers - 1 ? 0.1f : 1.0f);
+        for (uint32_t i = 0; i < kL.in[L] * kL.out[L]; ++i) {
+            const double u = (splitmix(st) >> 11) * (1.0 / 9007199254740992.0);
+            p[kL.w[L] + i] = static_cast<float>((2.0 * u - 1.0) * bound);
+        }
+    }
+    nb_status s = nb_cache_set_params(c, p.data());
+    if (s != NB_OK) { nb_cache_destroy(c); return s; }
+    *out = c;
+    return NB_OK;
+}
+
+nb_status nb_cache_destroy(nb_cache *c) {
+    if (!c) return NB_OK;
+    hipSetDevice(c->ctx->device);
+    if (c->stream) hipStreamSynchronize(c->stream);
+    void *ps[] = {c->params, c->adam_m, c->adam_v, c->grad, c->wbf, c->b5, c->raw, c->target, c->y, c->rgb, c->loss,
+                  c->x, c->dy};
+    for (void *p : ps) hipFree(p);
+    for (int L = 0; L < 5; ++L) { hipFree(c->h[L]); hipFree(c->dz[L]); }
+    if (c->stream) hipStreamDestroy(c->stream);
+    nb_context *ctx = c->ctx;
+    delete c;
+    context_release_child(ctx);
+    return NB_OK;
+}
+
+nb_status nb_cache_get_params(nb_cache *c, float *out) {
+    if (!c || !out) return set_error(NB_ERR_INVALID_ARGUMENT, "null argument");
+    C_HIP(hipSetDevice(c->ctx->device));
+    C_HIP(hipStreamSynchronize(c->stream));
+    C_HIP(hipMemcpy(out, c->params, kMlpParams * 4, hipMemcpyDeviceToHost));
+    return NB_OK;
+}
+
+nb_status nb_cache_set_params(nb_cache *c, const float *in) {
+    if (!c || !in) return set_error(NB_ERR_INVALID_ARGUMENT, "null argument");
+    C_HIP(hipSetDevice(c->ctx->device));
+    C_HIP(hipStreamSynchronize(c->stream));
+    C_HIP(hipMemcpy(c->params, in, kMlpParams * 4, hipMemcpyHostToDevice));
+    C_TRY(cache_refresh(c));
+    C_HIP(hipStreamSynchronize(c->stream));
+    return NB_OK;
+}
+
+nb_status nb_cache_encode(nb_cache *c, const float *raw, uint32_t n, uint16_t *out) {
+    if (!c || !raw || !out) return set_error(NB_ERR_INVALID_ARGUMENT, "null argument");
+    if (n > c->max_batch) return set_error(NB_ERR_INVALID_ARGUMENT, "batch %u > max_batch %u", n, c->max_batch);
+    C_HIP(hipSetDevice(c->ctx->device));
+    C_HIP(hipMemcpyAsync(c->raw, raw, static_cast<size_t>(n) * kMlpRaw * 4, hipMemcpyHostToDevice, c->stream));
+    C_TRY(mlp_launch_encode(c->dev(), c->raw, n, c->x, c->stream));
+    C_HIP(hipMemcpyAsync(out, c->x, static_cast<size_t>(n) * kMlpIn * 2, hipMemcpyDeviceToHost, c->stream));
+    C_HIP(hipStreamSynchronize(c->stream));
+    return NB_OK;
+}
+
+nb_status nb_cache_infer(nb_cache *c, const float *raw, uint32_t n, float *rgb) {
+    if (!c || !raw || !rgb) return set_error(NB_ERR_INVALID_ARGUMENT, "null argument");
+    if (n > c->max_batch) return set_error(NB_ERR_INVALID_ARGUMENT, "batch %u > max_batch %u", n, c->max_batch);
+    C_HIP(hipSetDevice(c->ctx->device));
+    C_HIP(hipMemcpyAsync(c->raw, raw, static_cast<size_t>(n) * kMlpRaw * 4, hipMemcpyHostToDevice, c->stream));
+    C_TRY(mlp_launch_infer(c->dev(), c->raw, n, c->rgb, c->stream));
+    C_HIP(hipMemcpyAsync(rgb, c->rgb, static_cast<size_t>(n) * 12, hipMemcpyDeviceToHost, c->stream));
+    C_HIP(hipStreamSynchronize(c->stream));
+    return NB_OK;
+}
+
+nb_status nb_cache_gradient(nb_cache *c, const float *raw, const float *target, uint32_t n, float *loss, float *grad) {
+    if (!c || !raw || !target || !loss || !grad) return set_error(NB_ERR_INVALID_ARGUMENT, "null argument");
+    if (n == 0 || n > c->max_batch) return set_error(NB_ERR_INVALID_ARGUMENT, "batch must be in 1..%u", c->max_batch);
+    C_HIP(hipSetDevice(c->ctx->device));
+    C_HIP(hipMemcpyAsync(c->raw, raw, static_cast<size_t>(n) * kMlpRaw * 4, hipMemcpyHostToDevice, c->stream));
+    C_HIP(hipMemcpyAsync(c->target, target, static_cast<size_t>(n) * 12, hipMemcpyHostToDevice, c->stream));
+    C_TRY(cache_forward_backward(c, n));
+    C_HIP(hipMemcpyAsync(loss, c->loss, 4, hipMemcpyDeviceToHost, c->stream));
+    C_HIP(hipMemcpyAsync(grad, c->grad, kMlpParams * 4, hipMemcpyDeviceToHost, c->stream));
+    C_HIP(hipStreamSynchronize(c->stream));
+    return NB_OK;
+}
+
+nb_status nb_cache_train(nb_cache *c, const float *raw, const float *target, uint32_t n, uint32_t batch, float lr,
+                         float *last_loss) {
+    if (!c || !raw || !target) return set_error(NB_ERR_INVALID_ARGUMENT, "null argument");
+    if (batch == 0 || batch > c->max_batch) return set_error(NB_ERR_INVALID_ARGUMENT, "batch must be in 1..%u", c->max_batch);
+    C_HIP(hipSetDevice(c->ctx->device));
+    for (uint32_t off = 0; off < n; off += batch) {
+        const uint32_t k = n - off < batch ? n - off : batch;
+        C_HIP(hipMemcpyAsync(c->raw, raw + static_cast<size_t>(off) * kMlpRaw, static_cast<size_t>(k) * kMlpRaw * 4,
+                             hipMemcpyHostToDevice, c->stream));
+        C_HIP(hipMemcpyAsync(c->target, target + static_cast<size_t>(off) * 3, static_cast<size_t>(k) * 12,
+                             hipMemcpyHostToDevice, c->stream));
+        C_TRY(cache_forward_backward(c, k));
+        C_TRY(cache_adam_step(c, lr));
+    }
+    if (last_loss) C_HIP(hipMemcpyAsync(last_loss, c->loss, 4, hipMemcpyDeviceToHost, c->stream));
+    C_HIP(hipStreamSynchronize(c->stream));
+    return NB_OK;
+}
+
+}  // extern "C"

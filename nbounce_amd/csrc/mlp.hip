@@ -1,0 +1,363 @@
+// Radiance-cache MLP kernels for gfx950: BF16 operands on the matrix cores
+// (v_mfma_f32_16x16x16_bf16), FP32 accumulation.
+//
+// Layout convention: every dense layer is computed transposed, Z^T = W H^T, with the weights
+// W (out x in, row-major) as the MFMA A operand and the activations as the B operand.  For
+// 16x16x16 MFMA, lane l holds A[l%16][4(l/16)+i], B[4(l/16)+i][l%16] and D[4(l/16)+i][l%16]:
+// the D tile of output features 16m..16m+15 for 16 batch rows is, lane for lane, exactly the B
+// tile of input features 16m..16m+15 of the next layer, so the fused inference kernel keeps a
+// row's activations in registers from the encoding to the output with no shuffles.
+#include "pt_device.h"
+
+namespace nb {
+namespace {
+using namespace dev;
+
+typedef short s4 __attribute__((ext_vector_type(4)));
+typedef float f4v __attribute__((ext_vector_type(4)));
+
+// binary32 -> bfloat16, round to nearest even
+__device__ __forceinline__ uint16_t f2bf(float f) {
+    uint32_t u = __float_as_uint(f);
+    u += 0x7FFFu + ((u >> 16) & 1u);
+    return static_cast<uint16_t>(u >> 16);
+}
+__device__ __forceinline__ float bf2f(uint16_t h) { return __uint_as_float(static_cast<uint32_t>(h) << 16); }
+
+__device__ __forceinline__ f4v mfma(s4 a, s4 b, f4v c) {
+    return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
+}
+
+__device__ __forceinline__ s4 load4(const uint16_t *p) {   // 4 consecutive bf16 (8-byte aligned)
+    uint2 v = *reinterpret_cast<const uint2 *>(p);
+    s4 r;
+    r[0] = static_cast<short>(v.x & 0xFFFFu); r[1] = static_cast<short>(v.x >> 16);
+    r[2] = static_cast<short>(v.y & 0xFFFFu); r[3] = static_cast<short>(v.y >> 16);
+    return r;
+}
+__device__ __forceinline__ s4 load4_strided(const uint16_t *p, uint32_t stride) {
+    s4 r;
+    for (int i = 0; i < 4; ++i) r[i] = static_cast<short>(p[i * stride]);
+    return r;
+}
+
+// Input encoding of one raw record r (pos3, dir3, normal3, roughness, albedo3, metallic):
+//   0..71   sin, cos of 2^f pi p_d, f = 0..11, p = position normalised to the scene box
+//   72..83  sin, cos of 2^f pi w_d, f = 0..1, w = direction
+//   84..86  normal, 87 roughness, 88..90 albedo, 91 metallic, 92..95 zero
+__device__ __forceinline__ float feature(const float *r, int f, const MlpDev &m) {
+    if (f < 84) {
+        float x, scale;
+        int q;
+        if (f < 72) {
+            const int d = f / 24;
+            q = f % 24;
+            x = (r[d] - m.scene_min[d]) * m.scene_inv_extent[d];
+        } else {
+            const int g = f - 72, d = g / 4;
+            q = g % 4;
+            x = r[3 + d];
+        }
+        scale = kPi * static_cast<float>(1u << (q >> 1));
+        float s, c;
+        sin_cos(x * scale, s, c);
+        return (q & 1) ? c : s;
+    }
+    if (f < 87) return r[6 + (f - 84)];
+    if (f == 87) return r[9];
+    if (f < 91) return r[10 + (f - 88)];
+    if (f == 91) return r[13];
+    return 0.0f;
+}
+
+constexpr int kPad = 4;                 // LDS row padding (bf16) against bank conflicts
+constexpr int kS0 = kMlpIn + kPad;      // 100
+constexpr int kSH = kMlpHidden + kPad;  // 68
+
+// Fused forward pass: weights staged once per workgroup in LDS (47 KB), each wave64 takes 16
+// rows at a time through all six layers in registers.
+__global__ __launch_bounds__(256) void mlp_infer_kernel(MlpDev m, const float *raw, uint32_t n, float *rgb) {
+    __shared__ uint16_t sw0[kMlpHidden * kS0];
+    __shared__ uint16_t swh[4][kMlpHidden * kSH];
+    __shared__ uint16_t sw5[kMlpOutPad * kSH];
+    __shared__ float sb[5 * kMlpHidden + kMlpOutPad];
+    for (int i = threadIdx.x; i < kMlpHidden * kMlpIn; i += blockDim.x)
+        sw0[(i / kMlpIn) * kS0 + i % kMlpIn] = m.w[0][i];
+    for (int L = 0; L < 4; ++L)
+        for (int i = threadIdx.x; i < kMlpHidden * kMlpHidden; i += blockDim.x)
+            swh[L][(i / kMlpHidden) * kSH + i % kMlpHidden] = m.w[L + 1][i];
+    for (int i = threadIdx.x; i < kMlpOutPad * kMlpHidden; i += blockDim.x)
+        sw5[(i / kMlpHidden) * kSH + i % kMlpHidden] = m.w[5][i];
+    for (int i = threadIdx.x; i < 5 * kMlpHidden + kMlpOutPad; i += blockDim.x)
+        sb[i] = i < 5 * kMlpHidden ? m.b[i / kMlpHidden][i % kMlpHidden] : m.b[5][i - 5 * kMlpHidden];
+    __syncthreads();
+
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int col = lane & 15, kq = lane >> 4;
+    const uint32_t waves = gridDim.x * 4u;
+    for (uint32_t base = (blockIdx.x * 4u + wave) * 16u; base < n; base += waves * 16u) {
+        const uint32_t row = base + col;
+        float r[kMlpRaw];
+        if (row < n) {
+            const float4 *p = reinterpret_cast<const float4 *>(raw + static_cast<size_t>(row) * kMlpRaw);
+            for (int q = 0; q < 4; ++q) {
+                float4 v = p[q];
+                r[4 * q] = v.x; r[4 * q + 1] = v.y; r[4 * q + 2] = v.z; r[4 * q + 3] = v.w;
+            }
+        } else {
+            for (int q = 0; q < kMlpRaw; ++q) r[q] = 0.0f;
+        }
+        s4 x[6];
+        for (int t = 0; t < 6; ++t)
+            for (int i = 0; i < 4; ++i) x[t][i] = static_cast<short>(f2bf(feature(r, 16 * t + 4 * kq + i, m)));
+        s4 h[4];
+        for (int mt = 0; mt < 4; ++mt) {
+            f4v acc = {0.0f, 0.0f, 0.0f, 0.0f};
+            for (int t = 0; t < 6; ++t) acc = mfma(load4(&sw0[(16 * mt + col) * kS0 + 16 * t + 4 * kq]), x[t], acc);
+            for (int i = 0; i < 4; ++i) h[mt][i] = static_cast<short>(f2bf(fmaxf(acc[i] + sb[16 * mt + 4 * kq + i], 0.0f)));
+        }
+        for (int L = 0; L < 4; ++L) {
+            s4 g[4];
+            for (int mt = 0; mt < 4; ++mt) {
+                f4v acc = {0.0f, 0.0f, 0.0f, 0.0f};
+                for (int t = 0; t < 4; ++t) acc = mfma(load4(&swh[L][(16 * mt + col) * kSH + 16 * t + 4 * kq]), h[t], acc);
+                const float *bl = sb + (L + 1) * kMlpHidden;
+                for (int i = 0; i < 4; ++i) g[mt][i] = static_cast<short>(f2bf(fmaxf(acc[i] + bl[16 * mt + 4 * kq + i], 0.0f)));
+            }
+            for (int mt = 0; mt < 4; ++mt) h[mt] = g[mt];
+        }
+        f4v acc = {0.0f, 0.0f, 0.0f, 0.0f};
+        for (int t = 0; t < 4; ++t) acc = mfma(load4(&sw5[col * kSH + 16 * t + 4 * kq]), h[t], acc);
+        if (kq == 0 && row < n) {
+            const float *b5 = sb + 5 * kMlpHidden;
+            for (int c = 0; c < kMlpOut; ++c) rgb[3 * static_cast<size_t>(row) + c] = acc[c] + b5[c];
+        }
+    }
+}
+
+__global__ void mlp_encode_kernel(MlpDev m, const float *raw, uint32_t n, uint16_t *x) {
+    const size_t total = static_cast<size_t>(n) * kMlpIn;
+    for (size_t i = blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x; i < total;
+         i += static_cast<size_t>(gridDim.x) * blockDim.x) {
+        const size_t row = i / kMlpIn;
+        const int f = static_cast<int>(i % kMlpIn);
+        float r[kMlpRaw];
+        for (int q = 0; q < kMlpRaw; ++q) r[q] = raw[row * kMlpRaw + q];
+        x[i] = f2bf(feature(r, f, m));
+    }
+}
+
+// Z = H_in W^T + b for 16-row tiles; W has round_up(out, 16) rows (padding rows zero).
+// Writes z (fp32, n x out) and/or h_out (bf16, n x out, ReLU applied when relu != 0).
+__global__ __launch_bounds__(256) void dense_fwd_kernel(const uint16_t *h_in, uint32_t k_in, const uint16_t *w,
+                                                        const float *b, uint32_t out, uint32_t n, float *z,
+                                                        uint16_t *h_out, int relu) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int col = lane & 15, kq = lane >> 4;
+    const uint32_t m_tiles = (out + 15) / 16, k_tiles = k_in / 16;
+    for (uint32_t tile = blockIdx.x * 4u + wave; tile * 16u < n; tile += gridDim.x * 4u) {
+        const uint32_t row = tile * 16u + col;
+        const bool valid = row < n;
+        for (uint32_t mt = 0; mt < m_tiles; ++mt) {
+            f4v acc = {0.0f, 0.0f, 0.0f, 0.0f};
+            for (uint32_t t = 0; t < k_tiles; ++t) {
+                s4 bop = valid ? load4(h_in + static_cast<size_t>(row) * k_in + 16 * t + 4 * kq) : s4{0, 0, 0, 0};
+                acc = mfma(load4(w + static_cast<size_t>(16 * mt + col) * k_in + 16 * t + 4 * kq), bop, acc);
+            }
+            if (!valid) continue;
+            for (int i = 0; i < 4; ++i) {
+                const uint32_t o = 16 * mt + 4 * kq + i;
+                if (o >= out) continue;
+                const float v = acc[i] + b[o];
+                if (z) z[static_cast<size_t>(row) * out + o] = v;
+                if (h_out) h_out[static_cast<size_t>(row) * out + o] = f2bf(relu ? fmaxf(v, 0.0f) : v);
+            }
+        }
+    }
+}
+
+// Relative L2 loss (y - t)^2 / (y^2 + 0.01) averaged over rows and channels, with y in the
+// denominator held constant; dy = d loss / d y, also as bf16 padded to 16 channels.
+__global__ __launch_bounds__(256) void loss_kernel(const float *y, const float *t, uint32_t n, float *dy,
+                                                   uint16_t *dy_bf, float *loss) {
+    __shared__ float part[256];
+    float acc = 0.0f;
+    const float scale = 1.0f / (3.0f * static_cast<float>(n));
+    for (uint32_t row = blockIdx.x * blockDim.x + threadIdx.x; row < n; row += gridDim.x * blockDim.x) {
+        for (int c = 0; c < 16; ++c) {
+            float g = 0.0f;
+            if (c < kMlpOut) {
+                const float yv = y[3 * static_cast<size_t>(row) + c], tv = t[3 * static_cast<size_t>(row) + c];
+                const float den = yv * yv + 0.01f;
+                const float d = yv - tv;
+                acc += d * d / den;
+                g = 2.0f * d / den * scale;
+                if (dy) dy[3 * static_cast<size_t>(row) + c] = g;
+            }
+            dy_bf[16 * static_cast<size_t>(row) + c] = f2bf(g);
+        }
+    }
+    part[threadIdx.x] = acc;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (threadIdx.x < s) part[threadIdx.x] += part[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) atomicAdd(loss, part[0] * scale);
+}
+
+// dH_prev^T = W^T dZ^T, masked by ReLU'(h_prev): dz (n x out_p bf16), w (out_p x k_in bf16).
+__global__ __launch_bounds__(256) void dense_bwd_data_kernel(const uint16_t *dz, uint32_t out_p, const uint16_t *w,
+                                                             uint32_t k_in, uint32_t n, const uint16_t *h_prev,
+                                                             uint16_t *dz_prev) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int col = lane & 15, kq = lane >> 4;
+    const uint32_t m_tiles = k_in / 16, k_tiles = out_p / 16;
+    for (uint32_t tile = blockIdx.x * 4u + wave; tile * 16u < n; tile += gridDim.x * 4u) {
+        const uint32_t row = tile * 16u + col;
+        const bool valid = row < n;
+        for (uint32_t mt = 0; mt < m_tiles; ++mt) {
+            f4v acc = {0.0f, 0.0f, 0.0f, 0.0f};
+            for (uint32_t t = 0; t < k_tiles; ++t) {
+                s4 a = load4_strided(w + static_cast<size_t>(16 * t + 4 * kq) * k_in + 16 * mt + col, k_in);
+                s4 bop = valid ? load4(dz + static_cast<size_t>(row) * out_p + 16 * t + 4 * kq) : s4{0, 0, 0, 0};
+                acc = mfma(a, bop, acc);
+            }
+            if (!valid) continue;
+            for (int i = 0; i < 4; ++i) {
+                const size_t idx = static_cast<size_t>(row) * k_in + 16 * mt + 4 * kq + i;
+                dz_prev[idx] = f2bf(bf2f(h_prev[idx]) > 0.0f ? acc[i] : 0.0f);
+            }
+        }
+    }
+}
+
+// dW += dZ^T H_prev over rows, db += column sums of dZ: one (16-output x 16-input) tile per wave
+// over a chunk of rows, then fp32 atomics.  Only outputs < out_valid are written.
+__global__ __launch_bounds__(256) void dense_bwd_weight_kernel(const uint16_t *dz, uint32_t out_p, uint32_t out_valid,
+                                                               const uint16_t *h_prev, uint32_t k_in, uint32_t n,
+                                                               uint32_t rows_per_chunk, float *dw, float *db) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int col = lane & 15, kq = lane >> 4;
+    const uint32_t m_tiles = out_p / 16, n_tiles = k_in / 16;
+    const uint32_t tiles = m_tiles * n_tiles;
+    const uint32_t chunks = (n + rows_per_chunk - 1) / rows_per_chunk;
+    for (uint32_t job = blockIdx.x * 4u + wave; job < tiles * chunks; job += gridDim.x * 4u) {
+        const uint32_t tile = job % tiles, chunk = job / tiles;
+        const uint32_t mt = tile / n_tiles, nt = tile % n_tiles;
+        const uint32_t r0 = chunk * rows_per_chunk, r1 = min(n, r0 + rows_per_chunk);
+        f4v acc = {0.0f, 0.0f, 0.0f, 0.0f};
+        float bsum = 0.0f;
+        for (uint32_t rb = r0; rb < r1; rb += 16) {
+            s4 a, bop;
+            for (int i = 0; i < 4; ++i) {
+                const uint32_t row = rb + 4 * kq + i;
+                const bool ok = row < r1;
+                a[i] = ok ? static_cast<short>(dz[static_cast<size_t>(row) * out_p + 16 * mt + col]) : short(0);
+                bop[i] = ok ? static_cast<short>(h_prev[static_cast<size_t>(row) * k_in + 16 * nt + col]) : short(0);
+                if (nt == 0 && ok) bsum += bf2f(static_cast<uint16_t>(a[i]));
+            }
+            acc = mfma(a, bop, acc);
+        }
+        for (int i = 0; i < 4; ++i) {
+            const uint32_t o = 16 * mt + 4 * kq + i;
+            if (o < out_valid) atomicAdd(&dw[static_cast<size_t>(o) * k_in + 16 * nt + col], acc[i]);
+        }
+        if (nt == 0) {
+            // lanes col, col+16, col+32, col+48 hold partial sums of output 16mt+col
+            bsum += __shfl_xor(bsum, 16, 64);
+            bsum += __shfl_xor(bsum, 32, 64);
+            const uint32_t o = 16 * mt + col;
+            if (kq == 0 && o < out_valid) atomicAdd(&db[o], bsum);
+        }
+    }
+}
+
+__global__ void adam_kernel(float *w, float *m, float *v, const float *g, uint32_t n, float lr, float b1, float b2,
+                            float eps, float c1, float c2) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const float gi = g[i];
+        const float mi = b1 * m[i] + (1.0f - b1) * gi;
+        const float vi = b2 * v[i] + (1.0f - b2) * gi * gi;
+        m[i] = mi;
+        v[i] = vi;
+        w[i] -= lr * (mi * c1) / (sqrtf(vi * c2) + eps);
+    }
+}
+
+__global__ void to_bf16_kernel(const float *src, uint16_t *dst, uint32_t n) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) dst[i] = f2bf(src[i]);
+}
+
+__global__ void fill_kernel(float *p, uint32_t n, float v) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = v;
+}
+
+nb_status launched(const char *what) {
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return set_error(NB_ERR_HIP, "%s launch: %s", what, hipGetErrorString(e));
+    return NB_OK;
+}
+
+uint32_t blocks_for(size_t items, uint32_t per_block, uint32_t cap = 4096) {
+    size_t b = (items + per_block - 1) / per_block;
+    return static_cast<uint32_t>(b < 1 ? 1 : (b > cap ? cap : b));
+}
+
+}  // namespace
+
+nb_status mlp_launch_encode(const MlpDev &m, const float *raw, uint32_t n, uint16_t *x, hipStream_t s) {
+    hipLaunchKernelGGL(mlp_encode_kernel, dim3(blocks_for(static_cast<size_t>(n) * kMlpIn, 256)), dim3(256), 0, s, m, raw, n, x);
+    return launched("mlp encode");
+}
+
+nb_status mlp_launch_infer(const MlpDev &m, const float *raw, uint32_t n, float *rgb, hipStream_t s) {
+    hipLaunchKernelGGL(mlp_infer_kernel, dim3(blocks_for(n, 64, 768)), dim3(256), 0, s, m, raw, n, rgb);
+    return launched("mlp infer");
+}
+
+nb_status mlp_launch_dense_fwd(const uint16_t *h_in, uint32_t k_in, const uint16_t *w, const float *b, uint32_t out,
+                               uint32_t n, float *z, uint16_t *h_out, int relu, hipStream_t s) {
+    hipLaunchKernelGGL(dense_fwd_kernel, dim3(blocks_for(n, 64)), dim3(256), 0, s, h_in, k_in, w, b, out, n, z, h_out, relu);
+    return launched("dense forward");
+}
+
+nb_status mlp_launch_loss(const float *y, const float *t, uint32_t n, float *dy, uint16_t *dy_bf, float *loss,
+                          hipStream_t s) {
+    hipLaunchKernelGGL(loss_kernel, dim3(blocks_for(n, 256, 1024)), dim3(256), 0, s, y, t, n, dy, dy_bf, loss);
+    return launched("loss");
+}
+
+nb_status mlp_launch_dense_bwd_data(const uint16_t *dz, uint32_t out_p, const uint16_t *w, uint32_t k_in, uint32_t n,
+                                    const uint16_t *h_prev, uint16_t *dz_prev, hipStream_t s) {
+    hipLaunchKernelGGL(dense_bwd_data_kernel, dim3(blocks_for(n, 64)), dim3(256), 0, s, dz, out_p, w, k_in, n, h_prev, dz_prev);
+    return launched("dense backward (data)");
+}
+
+nb_status mlp_launch_dense_bwd_weight(const uint16_t *dz, uint32_t out, const uint16_t *h_prev, uint32_t k_in,
+                                      uint32_t n, float *dw, float *db, hipStream_t s) {
+    const uint32_t out_p = (out + 15) / 16 * 16;
+    const uint32_t chunk = 1024;
+    const size_t jobs = static_cast<size_t>(out_p / 16) * (k_in / 16) * ((n + chunk - 1) / chunk);
+    hipLaunchKernelGGL(dense_bwd_weight_kernel, dim3(blocks_for(jobs, 4)), dim3(256), 0, s, dz, out_p, out, h_prev, k_in,
+                       n, chunk, dw, db);
+    return launched("dense backward (weights)");
+}
+
+nb_status mlp_launch_adam(float *w, float *m, float *v, const float *g, uint32_t n, float lr, float b1, float b2,
+                          float eps, float c1, float c2, hipStream_t s) {
+    hipLaunchKernelGGL(adam_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, s, w, m, v, g, n, lr, b1, b2, eps, c1, c2);
+    return launched("adam");
+}
+
+nb_status mlp_launch_to_bf16(const float *src, uint16_t *dst, uint32_t n, hipStream_t s) {
+    hipLaunchKernelGGL(to_bf16_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, s, src, dst, n);
+    return launched("to bf16");
+}
+
+nb_status mlp_launch_fill(float *p, uint32_t n, float v, hipStream_t s) {
+    hipLaunchKernelGGL(fill_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, s, p, n, v);
+    return launched("fill");
+}
+
+}  // namespace nb

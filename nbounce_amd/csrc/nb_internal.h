@@ -175,3 +175,35 @@ namespace nb {
 void context_add_child(nb_context *ctx);
 void context_release_child(nb_context *ctx);
 }  // namespace nb
+
+// ---- learned radiance cache (mlp.hip, cache.cpp) ------------------------------------------------
+namespace nb {
+// Network: 96 encoded inputs -> 5 hidden layers of 64 (ReLU) -> 3 outputs.  Weights follow the
+// (out x in) row-major convention; the last layer is padded to 16 output rows with zeros.
+constexpr int kMlpIn = 96, kMlpHidden = 64, kMlpOut = 3, kMlpLayers = 6, kMlpOutPad = 16;
+constexpr int kMlpRaw = 16;     // raw record: pos3, dir3, normal3, roughness, albedo3, metallic, 0, 0
+constexpr int kMlpWeights = kMlpHidden * kMlpIn + 4 * kMlpHidden * kMlpHidden + kMlpOut * kMlpHidden;
+constexpr int kMlpBiases = 5 * kMlpHidden + kMlpOut;
+constexpr int kMlpParams = kMlpWeights + kMlpBiases;
+
+struct MlpDev {
+    const uint16_t *w[kMlpLayers];   // bf16, (out x in) row-major; layer 5 padded to 16 rows
+    const float *b[kMlpLayers];      // fp32 biases (layer 5 padded to 16)
+    float scene_min[3], scene_inv_extent[3];
+};
+
+nb_status mlp_launch_encode(const MlpDev &m, const float *raw, uint32_t n, uint16_t *x, hipStream_t s);
+nb_status mlp_launch_infer(const MlpDev &m, const float *raw, uint32_t n, float *rgb, hipStream_t s);
+nb_status mlp_launch_dense_fwd(const uint16_t *h_in, uint32_t k_in, const uint16_t *w, const float *b, uint32_t out,
+                               uint32_t n, float *z, uint16_t *h_out, int relu, hipStream_t s);
+nb_status mlp_launch_loss(const float *y, const float *t, uint32_t n, float *dy, uint16_t *dy_bf, float *loss,
+                          hipStream_t s);
+nb_status mlp_launch_dense_bwd_data(const uint16_t *dz, uint32_t out, const uint16_t *w, uint32_t k_in, uint32_t n,
+                                    const uint16_t *h_prev, uint16_t *dz_prev, hipStream_t s);
+nb_status mlp_launch_dense_bwd_weight(const uint16_t *dz, uint32_t out, const uint16_t *h_prev, uint32_t k_in,
+                                      uint32_t n, float *dw, float *db, hipStream_t s);
+nb_status mlp_launch_adam(float *w, float *m, float *v, const float *g, uint32_t n, float lr, float b1, float b2,
+                          float eps, float c1, float c2, hipStream_t s);
+nb_status mlp_launch_to_bf16(const float *src, uint16_t *dst, uint32_t n, hipStream_t s);
+nb_status mlp_launch_fill(float *p, uint32_t n, float v, hipStream_t s);
+}  // namespace nb

@@ -346,3 +346,23 @@ SMALL = dict(width=256, height=256, spp=16, bounces=8, contribution_factor=4.0, 
              eye=(5.0, 5.0, 9.5), target=(5.0, 4.0, 0.0))
 LARGE = dict(width=1920, height=1080, spp=64, bounces=12, contribution_factor=4.0, env_face=256,
              eye=(-20.0, 110.0, -20.0), target=(100.0, 95.0, 100.0))
+
+# Learned-cache workload: the spheres-in-box scene at 1920x1080, one sample per frame for 240 frames
+# while the camera orbits; errors are measured at the orbit's final pose on every 8th pixel.
+CACHE = dict(width=1920, height=1080, frames=240, bounces=8, contribution_factor=4.0, env_face=128,
+             centre=(5.0, 3.0, 5.0), radius=3.5, height_y=6.0, eval_stride=8)
+
+
+def orbit_pose(k, frames=240, centre=(5.0, 3.0, 5.0), radius=3.5, height_y=6.0):
+    """Camera eye and target of orbit frame k (frame `frames` returns to frame 0)."""
+    a = 2.0 * math.pi * (k % frames) / frames
+    eye = (centre[0] + radius * math.cos(a), height_y, centre[2] + radius * math.sin(a))
+    return eye, tuple(centre)
+
+
+def eval_pixels(width=1920, height=1080, stride=8):
+    """Pixel centres of every `stride`-th pixel: (stride*i + stride/2, stride*j + stride/2)."""
+    xs = np.arange(stride // 2, width, stride)
+    ys = np.arange(stride // 2, height, stride)
+    X, Y = np.meshgrid(xs, ys)
+    return np.stack([X.ravel(), Y.ravel()], 1).astype(np.uint32)

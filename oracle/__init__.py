@@ -59,6 +59,8 @@ def _load():
         [ctypes.c_uint32] * 6 + [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int]
     lib.orc_render_mean_f64.argtypes = [ctypes.c_void_p] * 4 + [ctypes.c_uint32] * 2 + [ctypes.c_void_p] + \
         [ctypes.c_uint32] * 4 + [ctypes.c_void_p, ctypes.c_int]
+    lib.orc_render_pixels_mean_f64.argtypes = [ctypes.c_void_p] * 4 + [ctypes.c_uint32] * 2 + [ctypes.c_void_p] + \
+        [ctypes.c_uint32] * 2 + [ctypes.c_void_p] + [ctypes.c_uint32] * 3 + [ctypes.c_void_p, ctypes.c_int]
     lib.orc_sample_pixel.argtypes = [ctypes.c_void_p] * 4 + [ctypes.c_uint32] * 2 + [ctypes.c_void_p] + \
         [ctypes.c_uint32] * 4 + [ctypes.c_void_p, ctypes.c_void_p]
     lib.orc_primary_hits.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32,
@@ -245,4 +247,14 @@ def render_mean_f64(scene, cam, g, table, stride, env, width, height, first, n, 
     out = np.zeros((height, width, 3), dtype=np.float64)
     lib.orc_render_mean_f64(scene.ref, p(cam), p(g), p(t), stride, t.shape[0], env.ref, width, height, first, n,
                             p(out), threads)
+    return out
+
+
+def render_pixels_mean_f64(scene, cam, g, table, stride, env, width, height, xy, first, n, threads=1):
+    """float64 mean of samples first..first+n-1 at pixels xy (N x 2 uint32)."""
+    t = np.ascontiguousarray(table, dtype=np.float32)
+    xy = np.ascontiguousarray(xy, dtype=np.uint32)
+    out = np.zeros((len(xy), 3), dtype=np.float64)
+    lib.orc_render_pixels_mean_f64(scene.ref, p(cam), p(g), p(t), stride, t.shape[0], env.ref, width, height, p(xy),
+                                   len(xy), first, n, p(out), threads)
     return out

@@ -1132,3 +1132,26 @@ void orc_render_mean_f64(const orc_scene *sc, const orc_camera *cam, const orc_g
         }
     }
 }
+
+/* Mean of samples first..first+n-1 (float64 accumulation) at a list of pixels (x, y pairs). */
+void orc_render_pixels_mean_f64(const orc_scene *sc, const orc_camera *cam, const orc_globals *g0, const float *table,
+                                uint32_t stride, uint32_t rows, const orc_env *env, uint32_t w, uint32_t h,
+                                const uint32_t *xy, uint32_t n_pixels, uint32_t first, uint32_t n, double *out64,
+                                int threads) {
+    (void)threads;
+#ifdef _OPENMP
+    #pragma omp parallel for schedule(dynamic, 16) num_threads(threads > 0 ? threads : 1)
+#endif
+    for (int k = 0; k < (int)n_pixels; ++k) {
+        orc_counters c = {0, 0, 0, 0};
+        orc_globals g = *g0;
+        double acc[3] = {0, 0, 0};
+        for (uint32_t i = 0; i < n; ++i) {
+            g.sample = first + i;
+            float s[3];
+            orc_sample_pixel(sc, cam, &g, table, stride, rows, env, w, h, xy[2 * k], xy[2 * k + 1], s, &c);
+            acc[0] += s[0]; acc[1] += s[1]; acc[2] += s[2];
+        }
+        out64[3 * k] = acc[0] / n; out64[3 * k + 1] = acc[1] / n; out64[3 * k + 2] = acc[2] / n;
+    }
+}

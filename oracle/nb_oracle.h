@@ -149,6 +149,12 @@ void orc_render_mean_f64(const orc_scene *scene, const orc_camera *cam, const or
                          const orc_env *env, uint32_t width, uint32_t height,
                          uint32_t first_sample, uint32_t n_samples, double *out64, int threads);
 
+void orc_render_pixels_mean_f64(const orc_scene *scene, const orc_camera *cam, const orc_globals *g,
+                                const float *sobol_table, uint32_t sobol_stride, uint32_t sobol_rows,
+                                const orc_env *env, uint32_t width, uint32_t height,
+                                const uint32_t *xy, uint32_t n_pixels, uint32_t first_sample, uint32_t n_samples,
+                                double *out64, int threads);
+
 #ifdef __cplusplus
 }
 #endif

@@ -28,7 +28,7 @@ def test_library_exports_every_declared_symbol():
     lib = ctypes.CDLL(_lib.LIB_PATH)
     missing = [n for n in names if not hasattr(lib, n)]
     assert not missing, missing
-    assert set(names) <= set(_lib.SIGNATURES) | {"nb_last_error", "nb_version"}
+    assert set(names) <= set(_lib.SIGNATURES) | {"nb_last_error", "nb_version", "nb_cache_param_count"}
 
 
 def test_struct_layouts_match_the_reference_sizes():

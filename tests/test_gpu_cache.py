@@ -1,0 +1,121 @@
+"""GPU tests of the radiance-cache MLP kernels (BF16 MFMA, FP32 accumulation) against the
+float64 oracle (oracle/mlp.py) that rounds the same values to bfloat16."""
+import ctypes
+
+import numpy as np
+import pytest
+
+from oracle import mlp
+
+pytestmark = pytest.mark.gpu
+
+BOX_MIN, BOX_MAX = np.array([0, 0, 0], np.float32), np.array([10, 10, 10], np.float32)
+
+
+def _batch(rng, n):
+    raw = np.zeros((n, 16), dtype=np.float32)
+    raw[:, 0:3] = rng.uniform(0, 10, size=(n, 3))
+    d = rng.normal(size=(n, 3))
+    raw[:, 3:6] = d / np.linalg.norm(d, axis=1, keepdims=True)
+    nrm = rng.normal(size=(n, 3))
+    raw[:, 6:9] = nrm / np.linalg.norm(nrm, axis=1, keepdims=True)
+    raw[:, 9] = rng.uniform(0.05, 1, n)
+    raw[:, 10:13] = rng.uniform(0, 1, size=(n, 3))
+    raw[:, 13] = rng.uniform(0, 1, n) < 0.3
+    target = rng.uniform(0, 3, size=(n, 3)).astype(np.float32)
+    return raw, target
+
+
+@pytest.fixture(scope="module")
+def cache():
+    import nbounce_amd as nb
+    from nbounce_amd._lib import check, lib, ptr
+    ctx = nb.Context(0)
+    h = ctypes.c_void_p()
+    check(lib.nb_cache_create(ctx.handle, ptr(BOX_MIN), ptr(BOX_MAX), 7, 1 << 16, ctypes.byref(h)))
+    yield lib, ptr, check, h
+    lib.nb_cache_destroy(h)
+    ctx.close()
+
+
+def _encode(cache, raw):
+    lib, ptr, check, h = cache
+    out = np.zeros((len(raw), 96), dtype=np.uint16)
+    check(lib.nb_cache_encode(h, ptr(raw), len(raw), ptr(out)))
+    return (out.astype(np.uint32) << 16).view(np.float32).astype(np.float64)
+
+
+def _params(cache):
+    lib, ptr, check, h = cache
+    p = np.zeros(lib.nb_cache_param_count(), dtype=np.float32)
+    check(lib.nb_cache_get_params(h, ptr(p)))
+    return p
+
+
+def test_encoding_matches_oracle(cache):
+    print("case: MLP input encoding, 4096 records", flush=True)
+    raw, _ = _batch(np.random.default_rng(0), 4096)
+    got = _encode(cache, raw)
+    want = mlp.bf16(mlp.encode(raw, BOX_MIN, BOX_MAX))
+    ulp = np.abs(want) * 2.0 ** -7 + 1e-30
+    assert np.mean(got == want) > 0.995
+    assert np.all(np.abs(got - want) <= ulp + 1e-7)
+
+
+def test_fused_inference_matches_oracle(cache):
+    lib, ptr, check, h = cache
+    print("case: fused MFMA inference, 5000 rows", flush=True)
+    raw, _ = _batch(np.random.default_rng(1), 5000)
+    params = mlp.init_params(3)
+    params[np.arange(len(params)) % 7 == 0] += 0.05      # nonzero biases everywhere
+    check(lib.nb_cache_set_params(h, ptr(params)))
+    got = np.zeros((5000, 3), dtype=np.float32)
+    check(lib.nb_cache_infer(h, ptr(raw), 5000, ptr(got)))
+    x = _encode(cache, raw)
+    want, _, _ = mlp.forward(params.astype(np.float64), x, bf=True)
+    rel = np.abs(got - want) / (np.abs(want) + 1e-2)
+    assert np.median(rel) < 1e-4 and rel.max() < 2e-2
+
+
+def test_gradient_matches_oracle(cache):
+    lib, ptr, check, h = cache
+    print("case: MFMA forward + backward gradient, 3000 rows", flush=True)
+    raw, target = _batch(np.random.default_rng(2), 3000)
+    params = mlp.init_params(4)
+    params[np.arange(len(params)) % 5 == 0] += 0.03
+    check(lib.nb_cache_set_params(h, ptr(params)))
+    loss = ctypes.c_float()
+    grad = np.zeros(len(params), dtype=np.float32)
+    check(lib.nb_cache_gradient(h, ptr(raw), ptr(target), 3000, ctypes.byref(loss), ptr(grad)))
+    x = _encode(cache, raw)
+    want_loss, want_grad, _ = mlp.loss_and_grad(params.astype(np.float64), x, target.astype(np.float64), bf=True)
+    assert abs(loss.value - want_loss) <= 1e-4 * want_loss
+    off = 0
+    for L, (i, o) in enumerate(mlp.DIMS):
+        for size in (i * o, o):
+            g, w = grad[off:off + size].astype(np.float64), want_grad[off:off + size]
+            off += size
+            err = np.linalg.norm(g - w) / (np.linalg.norm(w) + 1e-20)
+            assert err < 1e-2, (L, size, err)
+
+
+def test_training_reduces_the_loss(cache):
+    lib, ptr, check, h = cache
+    print("case: Adam training on a smooth radiance field", flush=True)
+    rng = np.random.default_rng(5)
+    raw, _ = _batch(rng, 1 << 15)
+    p = raw[:, 0:3] / 10.0
+    target = np.stack([0.5 + 0.4 * np.sin(3 * p[:, 0]) * np.cos(2 * p[:, 2]),
+                       0.3 + 0.2 * p[:, 1],
+                       0.8 * raw[:, 10]], 1).astype(np.float32)
+    check(lib.nb_cache_set_params(h, ptr(mlp.init_params(6))))
+    l0 = ctypes.c_float()
+    g = np.zeros(lib.nb_cache_param_count(), dtype=np.float32)
+    check(lib.nb_cache_gradient(h, ptr(raw[:4096]), ptr(target[:4096]), 4096, ctypes.byref(l0), ptr(g)))
+    last = ctypes.c_float()
+    for _ in range(8):
+        check(lib.nb_cache_train(h, ptr(raw), ptr(target), len(raw), 1024, 3e-3, ctypes.byref(last)))
+    l1 = ctypes.c_float()
+    check(lib.nb_cache_gradient(h, ptr(raw[:4096]), ptr(target[:4096]), 4096, ctypes.byref(l1), ptr(g)))
+    print(f"loss {l0.value:.4f} -> {l1.value:.5f}")
+    assert l1.value < 0.05 * l0.value
