@@ -158,6 +158,94 @@ def serve_bench(args):
     print(json.dumps(out), flush=True)
 
 
+def cache_bench(args):
+    """Learned radiance cache on the spheres-in-box scene at 1920x1080, 8 bounces: 240 frames of
+    one sample while the camera orbits, each frame training the cache on `--train-paths` full
+    paths; then, at the orbit's final pose, relative MSE (every 8th pixel, against the committed
+    float64 4096-spp CPU-oracle reference) and device time of the plain tracer at 1..128 spp and
+    of the cache-assisted tracer (trained network, no further training) at 1..32 spp."""
+    import nbounce_amd as nb
+    from nbounce_amd import scenegen
+    from nbounce_amd.cache import RadianceCache, rel_mse
+    P = scenegen.CACHE
+    ref = np.load(os.path.join(ROOT, "tests", "golden", "cache_reference_spheres_1080p.npz"), allow_pickle=False)
+    xy, ref_mean = ref["xy"], ref["mean"].astype(np.float64)
+    v, i, pr = scenegen.spheres_in_box(0)
+    scene = nb.Scene(v, i, pr)
+    blas, tlas, inst = nb.build_scene(scene)            # the reference's per-primitive BLAS, as the oracle
+    ctx = nb.Context(0)
+    sb = nb.SceneBuffers(ctx, blas, tlas, inst, scene.vertices, scene.indices)
+    env = nb.EnvMap(ctx, scenegen.sky_cube(P["env_face"]))
+    W, H = P["width"], P["height"]
+
+    def tracer(k):
+        eye, target = scenegen.orbit_pose(k, P["frames"], P["centre"], P["radius"], P["height_y"])
+        cam = nb.CameraController(world_position=eye, target=target, aspect_ratio=W / H)
+        pt = nb.Pathtracer(ctx, sb, cam, env, W, H, resolution_factor=1.0)
+        pt.globals.bounces, pt.globals.contribution_factor = P["bounces"], P["contribution_factor"]
+        return pt
+
+    def err(pt):
+        img = pt.output_texture()
+        return rel_mse(img[xy[:, 1], xy[:, 0]], ref_mean)
+
+    cache = RadianceCache(ctx, [0, 0, 0], [10, 10, 10], seed=0, max_batch=1 << 16)
+    train_ms, losses = [], []
+    plain_orbit_ms = []
+    for k in range(P["frames"]):
+        pt = tracer(k)
+        losses.append(cache.render(pt, train_paths=args.train_paths, lr=args.lr))
+        train_ms.append(ctx.last_render_ms())
+        if k % 20 == 0:
+            pq = tracer(k)
+            pq.dispatch()
+            plain_orbit_ms.append(ctx.last_render_ms())
+    final = P["frames"]
+    plain = []
+    for n in (1, 2, 4, 8, 16, 32, 64, 128):
+        pt = tracer(final)
+        ctx.timer_start()
+        pt.dispatch_frames(n)
+        ms = ctx.timer_stop()
+        plain.append({"spp": n, "rel_mse": err(pt), "ms": round(ms, 3)})
+    cached = []
+    for n in (1, 2, 4, 8, 16, 32):
+        pt = tracer(final)
+        ctx.timer_start()
+        for _ in range(n):
+            cache.render(pt, train_paths=0)
+        ms = ctx.timer_stop()
+        cached.append({"spp": n, "rel_mse": err(pt), "ms": round(ms, 3)})
+    # plain spp (log-log interpolation) that reaches the 1-spp cache-assisted error
+    e1, t1 = cached[0]["rel_mse"], cached[0]["ms"]
+    xs = np.log([p["spp"] for p in plain])
+    ys = np.log([p["rel_mse"] for p in plain])
+    spp_eq = float(np.exp(np.interp(-np.log(e1), -ys, xs))) if e1 > plain[-1]["rel_mse"] else float("inf")
+    ms_per_spp = plain[-1]["ms"] / plain[-1]["spp"]
+    t_plain_eq = spp_eq * ms_per_spp
+    t_cache_train = float(np.mean(train_ms[-20:]))
+    out = {
+        "metric": "equal relative-MSE frame time, cache-assisted / plain",
+        "value": round(t1 / t_plain_eq, 4) if np.isfinite(t_plain_eq) else None,
+        "unit": "ratio (lower is better)",
+        "higher_is_better": False,
+        "with_training_ratio": round(t_cache_train / t_plain_eq, 4) if np.isfinite(t_plain_eq) else None,
+        "cache_1spp": {"rel_mse": e1, "ms": t1},
+        "plain_spp_at_equal_error": round(spp_eq, 3),
+        "plain_ms_per_spp": round(ms_per_spp, 4),
+        "plain_ms_per_frame_orbit": round(float(np.mean(plain_orbit_ms)), 4),
+        "cache_frame_with_training_ms": round(t_cache_train, 4),
+        "loss_first_last": [round(losses[0], 4), round(float(np.mean(losses[-10:])), 4)],
+        "plain": plain,
+        "cached": cached,
+        "n_gpus": 1,
+        "config": {"workload": "spheres-in-box scene, 1920x1080, 8 bounces, 240-frame orbit, 1 spp per frame",
+                   "train_paths_per_frame": args.train_paths, "lr": args.lr,
+                   "reference": "CPU oracle float64 mean of 4096 spp at every 8th pixel"},
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -169,7 +257,9 @@ def main():
     ap.add_argument("--reference-blas", action="store_true", help="rebuild the BLAS per instance as the reference")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
-    ap.add_argument("--mode", choices=["render", "serve"], default="render")
+    ap.add_argument("--mode", choices=["render", "serve", "cache"], default="render")
+    ap.add_argument("--train-paths", type=int, default=21845)
+    ap.add_argument("--lr", type=float, default=2e-3)
     ap.add_argument("--qps", type=float, default=20.0)
     ap.add_argument("--requests", type=int, default=200)
     ap.add_argument("--max-paths", type=int, default=1 << 23)
@@ -182,6 +272,9 @@ def main():
         sys.exit(2)
     if args.mode == "serve":
         serve_bench(args)
+        return
+    if args.mode == "cache":
+        cache_bench(args)
         return
 
     import nbounce_amd as nb

@@ -241,6 +241,16 @@ nb_status nb_cache_gradient(nb_cache *cache, const float *raw, const float *targ
                             float *grad);
 nb_status nb_cache_train(nb_cache *cache, const float *raw, const float *target, uint32_t n, uint32_t batch,
                          float lr, float *last_loss);
+/* One cache-assisted frame: one sample per pixel whose path ends at its second shading point
+ * (later after near-mirror bounces, at bounce 3 at the latest) with the radiance the cache
+ * predicts there, accumulated into the target with globals->weight as nb_render does.  When
+ * train_paths > 0, that many full paths of the plain integrator are traced first at a spread of
+ * pixels; their shading points (up to three per path) with the path's suffix radiance become a
+ * training set for Adam steps of 16384 records with learning rate lr.  *loss (optional) gets the
+ * last step's loss.  The cache must have been created on the same context. */
+nb_status nb_render_cached(nb_context *ctx, const nb_scene *scene, nb_target *target, const nb_camera *camera,
+                           const nb_globals *globals, const nb_envmap *env, nb_cache *cache, uint32_t train_paths,
+                           float lr, float *loss);
 
 #ifdef __cplusplus
 }

@@ -64,6 +64,11 @@ struct nb_cache {
     uint16_t *wbf = nullptr;     // bf16 weights, last layer padded
     float *b5 = nullptr;         // last-layer bias padded to 16
     uint32_t adam_t = 0;
+    // frame workspace (grows with the target)
+    size_t ws_pixels = 0;
+    float4 *direct = nullptr;
+    float *q_raw = nullptr, *q_thr = nullptr, *q_rgb = nullptr;
+    uint32_t *counts = nullptr;    // [0] queries, [1] training records
     // batch buffers
     float *raw = nullptr, *target = nullptr, *y = nullptr, *rgb = nullptr, *loss = nullptr;
     uint16_t *x = nullptr, *h[5] = {}, *dz[5] = {}, *dy = nullptr;
@@ -92,10 +97,10 @@ nb_status cache_refresh(nb_cache *c) {
 
 // Forward and backward over the batch in c->raw / c->target (n rows); gradients into c->grad,
 // mean loss into c->loss.  No parameter update.
-nb_status cache_forward_backward(nb_cache *c, uint32_t n) {
+nb_status cache_forward_backward(nb_cache *c, const float *raw, const float *target, uint32_t n) {
     const MlpDev m = c->dev();
     hipStream_t s = c->stream;
-    C_TRY(mlp_launch_encode(m, c->raw, n, c->x, s));
+    C_TRY(mlp_launch_encode(m, raw, n, c->x, s));
     const uint16_t *in = c->x;
     for (int L = 0; L < 5; ++L) {
         C_TRY(mlp_launch_dense_fwd(in, kL.in[L], m.w[L], m.b[L], kMlpHidden, n, nullptr, c->h[L], 1, s));
@@ -103,7 +108,7 @@ nb_status cache_forward_backward(nb_cache *c, uint32_t n) {
     }
     C_TRY(mlp_launch_dense_fwd(c->h[4], kMlpHidden, m.w[5], m.b[5], kMlpOut, n, c->y, nullptr, 0, s));
     C_TRY(mlp_launch_fill(c->loss, 1, 0.0f, s));
-    C_TRY(mlp_launch_loss(c->y, c->target, n, nullptr, c->dy, c->loss, s));
+    C_TRY(mlp_launch_loss(c->y, target, n, nullptr, c->dy, c->loss, s));
     C_TRY(mlp_launch_fill(c->grad, kMlpParams, 0.0f, s));
     // last layer
     C_TRY(mlp_launch_dense_bwd_weight(c->dy, kMlpOut, c->h[4], kMlpHidden, n, c->grad + kL.w[5], c->grad + kL.b[5], s));
@@ -152,12 +157,14 @@ nb_status nb_cache_create(nb_context *ctx, const float scene_min[3], const float
     c->max_batch = (max_batch + 15) / 16 * 16;
     for (int d = 0; d < 3; ++d) { c->scene_min[d] = scene_min[d]; c->scene_inv[d] = 1.0f / (scene_max[d] - scene_min[d]); }
     const size_t B = c->max_batch;
-    hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    c->stream = ctx->stream;      // the cache's work is ordered with the context's renders
+    hipError_t e = hipSuccess;
     auto alloc = [&](auto **p, size_t bytes) { if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void **>(p), bytes); };
     alloc(&c->params, kMlpParams * 4); alloc(&c->adam_m, kMlpParams * 4); alloc(&c->adam_v, kMlpParams * 4);
     alloc(&c->grad, kMlpParams * 4); alloc(&c->wbf, kBf16Weights * 2); alloc(&c->b5, kMlpOutPad * 4);
     alloc(&c->raw, B * kMlpRaw * 4); alloc(&c->target, B * 3 * 4); alloc(&c->y, B * 3 * 4); alloc(&c->rgb, B * 3 * 4);
     alloc(&c->loss, 4); alloc(&c->x, B * kMlpIn * 2); alloc(&c->dy, B * kMlpOutPad * 2);
+    alloc(&c->counts, 16);
     for (int L = 0; L < 5; ++L) { alloc(&c->h[L], B * kMlpHidden * 2); alloc(&c->dz[L], B * kMlpHidden * 2); }
     if (e == hipSuccess) e = hipMemset(c->wbf, 0, kBf16Weights * 2);
     if (e == hipSuccess) e = hipMemset(c->b5, 0, kMlpOutPad * 4);
@@ -186,12 +193,11 @@ nb_status nb_cache_create(nb_context *ctx, const float scene_min[3], const float
 nb_status nb_cache_destroy(nb_cache *c) {
     if (!c) return NB_OK;
     hipSetDevice(c->ctx->device);
-    if (c->stream) hipStreamSynchronize(c->stream);
+    hipStreamSynchronize(c->stream);
     void *ps[] = {c->params, c->adam_m, c->adam_v, c->grad, c->wbf, c->b5, c->raw, c->target, c->y, c->rgb, c->loss,
-                  c->x, c->dy};
+                  c->x, c->dy, c->direct, c->q_raw, c->q_thr, c->q_rgb, c->counts};
     for (void *p : ps) hipFree(p);
     for (int L = 0; L < 5; ++L) { hipFree(c->h[L]); hipFree(c->dz[L]); }
-    if (c->stream) hipStreamDestroy(c->stream);
     nb_context *ctx = c->ctx;
     delete c;
     context_release_child(ctx);
@@ -244,7 +250,7 @@ nb_status nb_cache_gradient(nb_cache *c, const float *raw, const float *target, 
     C_HIP(hipSetDevice(c->ctx->device));
     C_HIP(hipMemcpyAsync(c->raw, raw, static_cast<size_t>(n) * kMlpRaw * 4, hipMemcpyHostToDevice, c->stream));
     C_HIP(hipMemcpyAsync(c->target, target, static_cast<size_t>(n) * 12, hipMemcpyHostToDevice, c->stream));
-    C_TRY(cache_forward_backward(c, n));
+    C_TRY(cache_forward_backward(c, c->raw, c->target, n));
     C_HIP(hipMemcpyAsync(loss, c->loss, 4, hipMemcpyDeviceToHost, c->stream));
     C_HIP(hipMemcpyAsync(grad, c->grad, kMlpParams * 4, hipMemcpyDeviceToHost, c->stream));
     C_HIP(hipStreamSynchronize(c->stream));
@@ -262,11 +268,83 @@ nb_status nb_cache_train(nb_cache *c, const float *raw, const float *target, uin
                              hipMemcpyHostToDevice, c->stream));
         C_HIP(hipMemcpyAsync(c->target, target + static_cast<size_t>(off) * 3, static_cast<size_t>(k) * 12,
                              hipMemcpyHostToDevice, c->stream));
-        C_TRY(cache_forward_backward(c, k));
+        C_TRY(cache_forward_backward(c, c->raw, c->target, k));
         C_TRY(cache_adam_step(c, lr));
     }
     if (last_loss) C_HIP(hipMemcpyAsync(last_loss, c->loss, 4, hipMemcpyDeviceToHost, c->stream));
     C_HIP(hipStreamSynchronize(c->stream));
+    return NB_OK;
+}
+
+nb_status nb_render_cached(nb_context *ctx, const nb_scene *scene, nb_target *target, const nb_camera *camera,
+                           const nb_globals *g, const nb_envmap *env, nb_cache *c, uint32_t train_paths, float lr,
+                           float *loss) {
+    if (!ctx || !scene || !target || !camera || !g || !env || !c) return set_error(NB_ERR_INVALID_ARGUMENT, "null argument");
+    if (scene->ctx != ctx || target->ctx != ctx || env->ctx != ctx || c->ctx != ctx)
+        return set_error(NB_ERR_INVALID_ARGUMENT, "all objects must belong to the context");
+    if (!ctx->sobol) return set_error(NB_ERR_INVALID_ARGUMENT, "no Sobol table set (nb_sobol_set)");
+    if (g->sample == 0) return set_error(NB_ERR_INVALID_ARGUMENT, "samples are 1-based");
+    C_HIP(hipSetDevice(ctx->device));
+    const size_t npix = static_cast<size_t>(target->width) * target->height;
+    if (npix > c->ws_pixels) {
+        hipFree(c->direct); hipFree(c->q_raw); hipFree(c->q_thr); hipFree(c->q_rgb);
+        c->direct = nullptr; c->q_raw = c->q_thr = c->q_rgb = nullptr; c->ws_pixels = 0;
+        C_HIP(hipMalloc(&c->direct, npix * 16));
+        C_HIP(hipMalloc(&c->q_raw, npix * kMlpRaw * 4));
+        C_HIP(hipMalloc(&c->q_thr, npix * 16));
+        C_HIP(hipMalloc(&c->q_rgb, npix * 12));
+        c->ws_pixels = npix;
+    }
+    CacheFrame f{};
+    f.scene = scene->dev;
+    f.env = DevEnv{env->texels, env->n};
+    f.sobol = ctx->sobol;
+    f.sobol_entries = ctx->sobol_entries;
+    f.sobol_stride = ctx->sobol_stride;
+    std::memcpy(f.c2w, camera->clip_to_world, sizeof(f.c2w));
+    f.width = target->width;
+    f.height = target->height;
+    f.sample = g->sample;
+    f.weight = g->weight;
+    f.bounces = g->bounces;
+    f.contribution_factor = g->contribution_factor;
+    f.mirror_alpha = 0.05f;
+    f.max_cache_bounce = 3;
+    f.target = target->pixels;
+    f.direct = c->direct;
+    f.q_raw = c->q_raw;
+    f.q_thr = c->q_thr;
+    f.q_count = c->counts;
+    f.train_paths = train_paths;
+    f.train_offset = static_cast<uint32_t>((static_cast<uint64_t>(g->sample) * 2654435761ull) % npix);
+    f.train_sample = 1u + (g->sample + 511u) % 1023u;
+    f.t_capacity = c->max_batch;
+    f.t_raw = c->raw;
+    f.t_target = c->target;
+    f.t_count = c->counts + 1;
+    f.counters = ctx->counters;
+    f.tiles_x = target->width / 8;
+    f.n_tiles = (target->width / 8) * (target->height / 8);
+    f.tiles_per_xcd = (f.n_tiles + 7) / 8;
+    C_HIP(hipEventRecord(ctx->ev_start, ctx->stream));
+    if (train_paths > 0) {
+        C_TRY(cache_launch_train_paths(f, ctx->stream));
+        uint32_t n = 0;
+        C_HIP(hipMemcpyAsync(&n, c->counts + 1, 4, hipMemcpyDeviceToHost, ctx->stream));
+        C_HIP(hipStreamSynchronize(ctx->stream));
+        if (n > c->max_batch) n = c->max_batch;
+        const uint32_t batch = 16384;
+        for (uint32_t off = 0; off + 256 <= n; off += batch) {
+            const uint32_t k = n - off < batch ? n - off : batch;
+            C_TRY(cache_forward_backward(c, c->raw + static_cast<size_t>(off) * kMlpRaw, c->target + 3 * static_cast<size_t>(off), k));
+            C_TRY(cache_adam_step(c, lr));
+        }
+        if (loss) C_HIP(hipMemcpyAsync(loss, c->loss, 4, hipMemcpyDeviceToHost, ctx->stream));
+    }
+    C_TRY(cache_launch_frame(f, c->dev(), c->q_rgb, ctx->stream));
+    C_HIP(hipEventRecord(ctx->ev_stop, ctx->stream));
+    ctx->timed = true;
+    if (loss && train_paths > 0) C_HIP(hipStreamSynchronize(ctx->stream));
     return NB_OK;
 }
 

@@ -76,7 +76,9 @@ constexpr int kSH = kMlpHidden + kPad;  // 68
 
 // Fused forward pass: weights staged once per workgroup in LDS (47 KB), each wave64 takes 16
 // rows at a time through all six layers in registers.
-__global__ __launch_bounds__(256) void mlp_infer_kernel(MlpDev m, const float *raw, uint32_t n, float *rgb) {
+__global__ __launch_bounds__(256) void mlp_infer_kernel(MlpDev m, const float *raw, uint32_t n_max, const uint32_t *n_dev,
+                                                        float *rgb) {
+    const uint32_t n = n_dev ? min(*n_dev, n_max) : n_max;
     __shared__ uint16_t sw0[kMlpHidden * kS0];
     __shared__ uint16_t swh[4][kMlpHidden * kSH];
     __shared__ uint16_t sw5[kMlpOutPad * kSH];
@@ -312,7 +314,14 @@ nb_status mlp_launch_encode(const MlpDev &m, const float *raw, uint32_t n, uint1
 }
 
 nb_status mlp_launch_infer(const MlpDev &m, const float *raw, uint32_t n, float *rgb, hipStream_t s) {
-    hipLaunchKernelGGL(mlp_infer_kernel, dim3(blocks_for(n, 64, 768)), dim3(256), 0, s, m, raw, n, rgb);
+    hipLaunchKernelGGL(mlp_infer_kernel, dim3(blocks_for(n, 64, 768)), dim3(256), 0, s, m, raw, n,
+                       static_cast<const uint32_t *>(nullptr), rgb);
+    return launched("mlp infer");
+}
+
+nb_status mlp_launch_infer_count(const MlpDev &m, const float *raw, const uint32_t *n_dev, uint32_t n_max, float *rgb,
+                                 hipStream_t s) {
+    hipLaunchKernelGGL(mlp_infer_kernel, dim3(blocks_for(n_max, 64, 768)), dim3(256), 0, s, m, raw, n_max, n_dev, rgb);
     return launched("mlp infer");
 }
 

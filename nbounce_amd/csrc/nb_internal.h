@@ -192,6 +192,34 @@ struct MlpDev {
     float scene_min[3], scene_inv_extent[3];
 };
 
+// One frame of cache-assisted rendering plus the training paths traced beside it.
+struct CacheFrame {
+    DevScene scene;
+    DevEnv env;
+    const float4 *sobol;
+    uint32_t sobol_entries, sobol_stride;
+    float c2w[16];
+    uint32_t width, height;
+    uint32_t sample;
+    float weight;
+    uint32_t bounces;
+    float contribution_factor;
+    float mirror_alpha;          // lobes with GGX alpha below this do not end the path in the cache
+    uint32_t max_cache_bounce;   // the path ends in the cache at this bounce at the latest
+    float4 *target, *direct;
+    float *q_raw, *q_thr;        // queries: raw records, (throughput rgb, pixel bits)
+    uint32_t *q_count;
+    uint32_t train_paths, train_offset, train_sample, t_capacity;
+    float *t_raw, *t_target;
+    uint32_t *t_count;
+    unsigned long long *counters;
+    uint32_t tiles_x, n_tiles, tiles_per_xcd;
+};
+
+nb_status cache_launch_frame(const CacheFrame &p, const MlpDev &m, float *rgb, hipStream_t s);
+nb_status cache_launch_train_paths(const CacheFrame &p, hipStream_t s);
+nb_status mlp_launch_infer_count(const MlpDev &m, const float *raw, const uint32_t *n_dev, uint32_t n_max, float *rgb,
+                                 hipStream_t s);
 nb_status mlp_launch_encode(const MlpDev &m, const float *raw, uint32_t n, uint16_t *x, hipStream_t s);
 nb_status mlp_launch_infer(const MlpDev &m, const float *raw, uint32_t n, float *rgb, hipStream_t s);
 nb_status mlp_launch_dense_fwd(const uint16_t *h_in, uint32_t k_in, const uint16_t *w, const float *b, uint32_t out,

@@ -339,18 +339,29 @@ struct PathConst {
 
 enum StepResult { kContinue = 0, kDone = 1 };
 
-// One iteration of the bounce loop of sample_rendering_eq (pathtracing.wgsl:164-239) with
-// intersect_scene (raytracing_sw.wgsl:138-173) inlined.  Either finishes the path (kDone,
-// radiance in `out`) or updates `ray` and `thr` for the next bounce (kContinue).
+// Shading point of a non-emissive hit, as the radiance cache sees it.
+struct Vertex {
+    F3 pos, wo, n, albedo;
+    float roughness, metallic;
+    float lobe_alpha;       // alpha of the sampled lobe: the GGX alpha if specular, 1 if diffuse
+};
+
+// Closest hit of `ray`, counted as one intersect_scene call.
 template <int K>
-__device__ StepResult bounce_step(const PathConst &pc, uint32_t bounce, Ray &ray, F3 &thr, F3 &out, Stack<K> &st,
-                                  LaneCounters &c) {
-    const DevScene &S = pc.scene;
-    Hit h;
-    trace<K>(S, ray, h, st, c);
+__device__ __forceinline__ void trace_counted(const PathConst &pc, const Ray &ray, Hit &h, Stack<K> &st,
+                                              LaneCounters &c) {
+    trace<K>(pc.scene, ray, h, st, c);
     ++c.rays;
-    if (h.t == kNoHit) { out = mul(thr, env_sample(pc.env, ray.d)); return kDone; }
-    ++c.hits;
+    if (h.t != kNoHit) ++c.hits;
+}
+
+// Shading of hit h of `ray` (the body of the bounce loop after intersect_scene,
+// pathtracing.wgsl:172-238).  kDone with `out` for emitters and Russian-roulette deaths,
+// otherwise the next ray and throughput.  With kRecord, `vtx` receives the shading point.
+template <bool kRecord>
+__device__ StepResult shade(const PathConst &pc, uint32_t bounce, const Hit &h, Ray &ray, F3 &thr, F3 &out,
+                            Vertex &vtx) {
+    const DevScene &S = pc.scene;
     const float4 *sh = S.ishade + 6 * static_cast<size_t>(h.inst);
     float4 s0 = sh[0], s1 = sh[1], s2 = sh[2], s3 = sh[3], s4 = sh[4], s5 = sh[5];
     F3 albedo = f3(s3.w, s4.w, s5.w);
@@ -374,6 +385,14 @@ __device__ StepResult bounce_step(const PathConst &pc, uint32_t bounce, Ray &ray
     float alpha = roughness * roughness;
     float alpha2 = alpha * alpha;
     F3 n = normalize(nrm);
+    if (kRecord) {
+        vtx.pos = pos;
+        vtx.wo = normalize(f3(-ray.d.x, -ray.d.y, -ray.d.z));
+        vtx.n = n;
+        vtx.albedo = albedo;
+        vtx.roughness = roughness;
+        vtx.metallic = metallic;
+    }
     uint32_t e0 = pc.sample * pc.sobol.stride + 1u + bounce * 2u;
     float4 sob0 = sobol_fetch(pc.sobol, e0, pc.shift), sob1 = sobol_fetch(pc.sobol, e0 + 1u, pc.shift);
     F3 wo = normalize(f3(-ray.d.x, -ray.d.y, -ray.d.z));
@@ -397,7 +416,9 @@ __device__ StepResult bounce_step(const PathConst &pc, uint32_t bounce, Ray &ray
         float den = (1.0f + ll) + lv;
         F3 spec = f3((F.x * (1.0f + lv)) / den, (F.y * (1.0f + lv)) / den, (F.z * (1.0f + lv)) / den);
         thr = mul(thr, divs(spec, p_spec));
+        if (kRecord) vtx.lobe_alpha = alpha;
     } else {
+        if (kRecord) vtx.lobe_alpha = 1.0f;
         F3 nt = normalize(sub(tan, scale(dot(tan, n), n)));
         F3 b = cross(n, nt);
         float phi = kTwoPi * sob1.y;
@@ -428,6 +449,19 @@ __device__ StepResult bounce_step(const PathConst &pc, uint32_t bounce, Ray &ray
     ray.d = wi;
     ray.inv = rcp3(wi);
     return kContinue;
+}
+
+// One iteration of the bounce loop of sample_rendering_eq (pathtracing.wgsl:164-239) with
+// intersect_scene (raytracing_sw.wgsl:138-173) inlined.  Either finishes the path (kDone,
+// radiance in `out`) or updates `ray` and `thr` for the next bounce (kContinue).
+template <int K, bool kRecord = false>
+__device__ __forceinline__ StepResult bounce_step(const PathConst &pc, uint32_t bounce, Ray &ray, F3 &thr, F3 &out,
+                                                  Stack<K> &st, LaneCounters &c, Vertex *vtx = nullptr) {
+    Hit h;
+    trace_counted<K>(pc, ray, h, st, c);
+    if (h.t == kNoHit) { out = mul(thr, env_sample(pc.env, ray.d)); return kDone; }
+    Vertex dummy;
+    return shade<kRecord>(pc, bounce, h, ray, thr, out, kRecord ? *vtx : dummy);
 }
 
 // One whole path (sample_rendering_eq, pathtracing.wgsl:161-241).

@@ -119,3 +119,65 @@ def test_training_reduces_the_loss(cache):
     check(lib.nb_cache_gradient(h, ptr(raw[:4096]), ptr(target[:4096]), 4096, ctypes.byref(l1), ptr(g)))
     print(f"loss {l0.value:.4f} -> {l1.value:.5f}")
     assert l1.value < 0.05 * l0.value
+
+
+def test_trained_cache_beats_the_plain_tracer_at_one_sample():
+    """What the trained network gains: at one sample per pixel the cache-assisted image is
+    closer to a 1024-spp reference (plain tracer) than the plain 1-spp image, and than the same
+    method with an untrained network."""
+    import math
+    import nbounce_amd as nb
+    from nbounce_amd import scenegen
+    from nbounce_amd.cache import RadianceCache, rel_mse
+    from tests import _workloads as Wl
+    print("case: cache-assisted vs plain at 1 spp, 192x112 spheres scene", flush=True)
+    w_, h_ = 192, 112
+    ctx = nb.Context(0)
+    wl = Wl.small()
+    sb = nb.SceneBuffers(ctx, wl["blas"], wl["tlas"], wl["instances"], wl["vertices"], wl["indices"])
+    env = nb.EnvMap(ctx, scenegen.sky_cube(64))
+    eye, target = scenegen.orbit_pose(0)
+    cam = nb.CameraController(world_position=eye, target=target, aspect_ratio=w_ / h_)
+
+    def tracer():
+        return nb.Pathtracer(ctx, sb, cam, env, w_, h_, resolution_factor=1.0)
+
+    ref_pt = tracer()
+    ref_pt.dispatch_frames(1023)
+    ref = ref_pt.output_texture()
+    plain = []
+    for k in range(4):
+        pt = tracer()
+        pt.globals.sample = 100 + 10 * k
+        pt.globals.weight = 0.0
+        pt.dispatch()
+        pt.globals.sample = 0
+        img = pt.output_texture()
+        plain.append(rel_mse(img, ref))
+
+    def cache_err(cache):
+        errs = []
+        for k in range(4):
+            pt = tracer()
+            pt.globals.sample = 300 + 10 * k
+            cache.render(pt, train_paths=0)
+            errs.append(rel_mse(pt.output_texture(), ref))
+        return float(np.mean(errs))
+
+    untrained = RadianceCache(ctx, [0, 0, 0], [10, 10, 10], seed=1)
+    e_untrained = cache_err(untrained)
+    cache = RadianceCache(ctx, [0, 0, 0], [10, 10, 10], seed=1)
+    train_pt = tracer()
+    losses = []
+    for f in range(160):
+        if train_pt.sample_count() >= 1000:
+            train_pt.invalidate()
+        losses.append(cache.render(train_pt, train_paths=8192, lr=2e-3))
+    e_cache = cache_err(cache)
+    e_plain = float(np.mean(plain))
+    print(f"relMSE plain 1spp {e_plain:.4f}, cache 1spp {e_cache:.4f}, untrained cache {e_untrained:.4f}, "
+          f"loss {losses[0]:.3f} -> {np.mean(losses[-10:]):.3f}")
+    assert math.isfinite(e_cache)
+    assert e_cache < 0.7 * e_plain
+    assert e_cache < 0.5 * e_untrained
+    cache.close(); untrained.close()
