@@ -35,14 +35,15 @@ class RadianceCache:
         check(lib.nb_cache_infer(self.handle, ptr(raw), len(raw), ptr(out)))
         return out
 
-    def render(self, pathtracer, train_paths=0, lr=1e-3):
+    def render(self, pathtracer, train_paths=0, lr=1e-3, weight=None):
         """One cache-assisted dispatch of `pathtracer` (same sample counting as dispatch()),
-        training on `train_paths` full paths first.  Returns the last training loss or None."""
+        training on `train_paths` full paths first.  `weight` overrides the accumulation weight
+        1/sample (1.0 makes a fresh one-sample image).  Returns the last training loss or None."""
         pt = pathtracer
         if pt.globals.sample >= pt.max_sample_count:
             return None
         pt.globals.sample += 1
-        pt.globals.weight = float(np.float32(1.0) / np.float32(pt.globals.sample))
+        pt.globals.weight = float(np.float32(1.0) / np.float32(pt.globals.sample)) if weight is None else weight
         g = pt.globals.array()
         loss = ctypes.c_float(-1.0)
         check(lib.nb_render_cached(self.ctx.handle, pt.scene.handle, pt.output.handle, ptr(pt._camera_data), ptr(g),

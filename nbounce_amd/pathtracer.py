@@ -292,12 +292,13 @@ class Pathtracer:
     def invalidate(self):
         self.globals.sample = 0
 
-    def dispatch(self):
-        """pathtracer.rs:225-239: +1 sample per pixel; no-op once max_sample_count is reached."""
+    def dispatch(self, weight=None):
+        """pathtracer.rs:225-239: +1 sample per pixel; no-op once max_sample_count is reached.
+        `weight` overrides the accumulation weight 1/sample (1.0 makes a fresh one-sample image)."""
         if self.globals.sample >= self.max_sample_count:
             return
         self.globals.sample += 1
-        self.globals.weight = float(np.float32(1.0) / np.float32(self.globals.sample))
+        self.globals.weight = float(np.float32(1.0) / np.float32(self.globals.sample)) if weight is None else weight
         g = self.globals.array()
         check(lib.nb_render(self.ctx.handle, self.scene.handle, self.output.handle, ptr(self._camera_data), ptr(g),
                             self.envmap.handle))

@@ -149,9 +149,7 @@ def test_trained_cache_beats_the_plain_tracer_at_one_sample():
     for k in range(4):
         pt = tracer()
         pt.globals.sample = 100 + 10 * k
-        pt.globals.weight = 0.0
-        pt.dispatch()
-        pt.globals.sample = 0
+        pt.dispatch(weight=1.0)
         img = pt.output_texture()
         plain.append(rel_mse(img, ref))
 
@@ -160,7 +158,7 @@ def test_trained_cache_beats_the_plain_tracer_at_one_sample():
         for k in range(4):
             pt = tracer()
             pt.globals.sample = 300 + 10 * k
-            cache.render(pt, train_paths=0)
+            cache.render(pt, train_paths=0, weight=1.0)
             errs.append(rel_mse(pt.output_texture(), ref))
         return float(np.mean(errs))
 
