@@ -33,7 +33,9 @@ typedef enum {
     NB_ERR_HIP = 3,
     NB_ERR_BAD_LAYOUT = 4,
     NB_ERR_UNSUPPORTED = 5,
-    NB_ERR_BUDGET = 6
+    NB_ERR_BUDGET = 6,
+    NB_ERR_FORMAT = 7,      /* a scene or image file was rejected (MeshError, scene.rs:69-98) */
+    NB_ERR_IO = 8           /* a file could not be read or written */
 } nb_status;
 
 /* ---- reference byte layouts ---------------------------------------------------------- */
@@ -87,6 +89,30 @@ nb_status nb_context_create(int device, nb_context **out);
 nb_status nb_context_destroy(nb_context *ctx);
 nb_status nb_context_synchronize(nb_context *ctx);
 
+/* ---- host scene and glTF ingestion: Scene (src/pathtracing/scene.rs:100-105) and
+ * Scene::parse_gltf (scene.rs:108-217) -----------------------------------------------------
+ * A host scene holds the reference's three arrays (Vertex, u32 indices, Primitive).  Each parse
+ * appends one glTF 2.0 document (.glb, or .gltf with data-URI or file buffers): every mesh
+ * primitive's vertices and offset indices, then one Primitive per mesh primitive of every node
+ * that has a mesh, with the node's local matrix (parents ignored, as in the reference).  A
+ * primitive that is not a triangle list or lacks POSITION, NORMAL, TANGENT or indices fails
+ * with NB_ERR_FORMAT and the reference's MeshError text; the scene is then left unchanged.
+ * flags: NB_GLTF_SHARE_GEOMETRY appends the geometry of a mesh primitive once per distinct
+ * accessor set, so primitives that reuse accessors share one index range (pair it with
+ * NB_BUILD_SHARE_BLAS); the reference appends a copy per mesh primitive. */
+#define NB_GLTF_SHARE_GEOMETRY 1u
+typedef struct nb_host_scene nb_host_scene;
+nb_status nb_host_scene_create(nb_host_scene **out);
+nb_status nb_host_scene_destroy(nb_host_scene *scene);
+nb_status nb_host_scene_parse_gltf(nb_host_scene *scene, const char *path, uint32_t flags);
+/* The same from memory; relative buffer URIs resolve against base_dir (NULL: none allowed). */
+nb_status nb_host_scene_parse_gltf_memory(nb_host_scene *scene, const void *data, size_t size,
+                                          const char *base_dir, uint32_t flags);
+nb_status nb_host_scene_counts(const nb_host_scene *scene, uint32_t *n_vertices, uint32_t *n_indices,
+                               uint32_t *n_prims);
+nb_status nb_host_scene_copy(const nb_host_scene *scene, nb_vertex *vertices, uint32_t *indices,
+                             nb_primitive *prims);
+
 /* ---- host-side scene build ------------------------------------------------------------- */
 /* SceneBuffers::from_scene (src/pathtracing/scene.rs:280-307) without the upload: builds one
  * BLAS per primitive into one shared node array (bvh.rs:162-190), the instance bounds
@@ -107,7 +133,7 @@ nb_status nb_build_scene(const nb_vertex *vertices, uint32_t n_vertices,
 
 /* ---- device scene: group-1 bindings blas, tlas, instances, vertices, indices ------------
  * (src/pathtracing/raytracing_sw.wgsl:33-37, uploaded at scene.rs:309-341).  The arrays are
- * validated (indices in range, children after parents, depth <= 48) and re-laid out for the
+ * validated (indices in range, children after parents, depth <= 32) and re-laid out for the
  * GPU; the caller's arrays are not kept. */
 nb_status nb_scene_create(nb_context *ctx,
                           const nb_bvh_node *blas, uint32_t n_blas,

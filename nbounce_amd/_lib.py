@@ -25,6 +25,12 @@ SIGNATURES = {
     "nb_context_synchronize": [_p],
     "nb_build_scene_capacity": [_u32, _p, _u32, _u32, _ptr(_u32), _ptr(_u32)],
     "nb_build_scene": [_p, _u32, _p, _u32, _p, _u32, _u32, _p, _u32, _ptr(_u32), _p, _u32, _ptr(_u32), _p],
+    "nb_host_scene_create": [_c.POINTER(_p)],
+    "nb_host_scene_destroy": [_p],
+    "nb_host_scene_parse_gltf": [_p, _c.c_char_p, _u32],
+    "nb_host_scene_parse_gltf_memory": [_p, _p, _c.c_size_t, _c.c_char_p, _u32],
+    "nb_host_scene_counts": [_p, _ptr(_u32), _ptr(_u32), _ptr(_u32)],
+    "nb_host_scene_copy": [_p, _p, _p, _p],
     "nb_scene_create": [_p, _p, _u32, _p, _u32, _p, _u32, _p, _u32, _p, _u32, _c.POINTER(_p)],
     "nb_scene_destroy": [_p],
     "nb_scene_device_bytes": [_p, _ptr(_c.c_uint64)],

@@ -79,8 +79,8 @@ __global__ __launch_bounds__(64) void cache_query_kernel(CacheFrame p) {
     const uint32_t x = (tile % p.tiles_x) * 8u + (lane & 7u), y = (tile / p.tiles_x) * 8u + (lane >> 3);
     const uint32_t pixel = y * p.width + x;
     Stack<kCK> st;
-    st.ref = s_ref + lane;
-    st.dist = s_dist + lane;
+    st.ref = (lds_u32 *)(s_ref + lane);
+    st.dist = (lds_f32 *)(s_dist + lane);
     LaneCounters c{0, 0, 0, 0, 0};
     PathConst pc;
     pc.scene = p.scene;
@@ -128,8 +128,8 @@ __global__ __launch_bounds__(64) void cache_train_paths_kernel(CacheFrame p) {
     __shared__ float s_dist[kCK * 64];
     const uint32_t lane = threadIdx.x;
     Stack<kCK> st;
-    st.ref = s_ref + lane;
-    st.dist = s_dist + lane;
+    st.ref = (lds_u32 *)(s_ref + lane);
+    st.dist = (lds_f32 *)(s_dist + lane);
     LaneCounters c{0, 0, 0, 0, 0};
     const uint32_t npix = p.width * p.height;
     const uint32_t stride = npix / p.train_paths > 0 ? npix / p.train_paths : 1u;

@@ -99,11 +99,15 @@ struct LaneCounters { uint32_t rays, aabb, tri, inst, hits; };
 
 struct Hit { float t, u, v; uint32_t tri, inst; };
 
+// LDS (address space 3) views, so stack accesses compile to ds_read / ds_write.
+typedef __attribute__((address_space(3))) uint32_t lds_u32;
+typedef __attribute__((address_space(3))) float lds_f32;
+
 // Traversal stack: top K entries per lane in LDS, older entries in private memory.
 template <int K>
 struct Stack {
-    uint32_t *ref;  // LDS, slot s of this lane at ref[s * 64]
-    float *dist;
+    lds_u32 *ref;   // slot s of this lane at ref[s * 64]
+    lds_f32 *dist;
     uint32_t sp, lo;
     uint2 spill[kStackEntries];
 
@@ -142,7 +146,7 @@ __device__ __forceinline__ uint32_t leaf_count(uint32_t ref, const uint32_t *sid
 // Closest hit of `world` over the two-level BVH (intersect_TLAS / intersect_BLAS,
 // raytracing_sw.wgsl:176-319).
 template <int K>
-__device__ void trace(const DevScene &S, const Ray &world, Hit &hit, Stack<K> &st, LaneCounters &c) {
+__device__ __forceinline__ void trace(const DevScene &S, const Ray &world, Hit &hit, Stack<K> &st, LaneCounters &c) {
     hit.t = kNoHit;
     hit.tri = 0; hit.inst = 0; hit.u = 0.0f; hit.v = 0.0f;
     if (S.tlas_root == kNoRecord) return;

@@ -77,8 +77,8 @@ __global__ __launch_bounds__(64) void serve_bounce_kernel(const ServeStep *step,
     const float4 *in = S.queue[level & 1u];
     float4 *out = S.queue[(level + 1u) & 1u];
     Stack<kServeK> st;
-    st.ref = s_ref + lane;
-    st.dist = s_dist + lane;
+    st.ref = (lds_u32 *)(s_ref + lane);
+    st.dist = (lds_f32 *)(s_dist + lane);
     LaneCounters c{0, 0, 0, 0, 0};
     uint32_t samples_done = 0;
     const SobolView sob{S.sobol, S.sobol_entries, S.sobol_stride};

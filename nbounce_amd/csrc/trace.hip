@@ -56,8 +56,8 @@ __global__ __launch_bounds__(64) void render_kernel(RenderParams p) {
     const uint32_t lane = threadIdx.x;
     const uint32_t x = (tile % p.tiles_x) * 8u + (lane & 7u), y = (tile / p.tiles_x) * 8u + (lane >> 3);
     Stack<K> st;
-    st.ref = s_ref + lane;
-    st.dist = s_dist + lane;
+    st.ref = (lds_u32 *)(s_ref + lane);
+    st.dist = (lds_f32 *)(s_dist + lane);
     LaneCounters c{0, 0, 0, 0, 0};
     float4 *px = p.target + static_cast<size_t>(y) * p.width + x;
     PathConst pc;
@@ -92,8 +92,8 @@ __global__ __launch_bounds__(64) void primary_hits_kernel(RenderParams p) {
     const uint32_t lane = threadIdx.x;
     const uint32_t x = (tile % p.tiles_x) * 8u + (lane & 7u), y = (tile / p.tiles_x) * 8u + (lane >> 3);
     Stack<K> st;
-    st.ref = s_ref + lane;
-    st.dist = s_dist + lane;
+    st.ref = (lds_u32 *)(s_ref + lane);
+    st.dist = (lds_f32 *)(s_dist + lane);
     LaneCounters c{0, 0, 0, 0, 0};
     const float4 shift = hash4f(x, y, x, y);
     float4 jit = sobol_fetch(SobolView{p.sobol, p.sobol_entries, p.sobol_stride}, p.first_sample * p.sobol_stride, shift);

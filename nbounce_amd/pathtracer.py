@@ -64,10 +64,40 @@ class Context:
 class Scene:
     """In-memory scene (src/pathtracing/scene.rs:100-105): vertices, indices, primitives."""
 
-    def __init__(self, vertices, indices, primitives):
-        self.vertices = np.ascontiguousarray(vertices, dtype=layout.VERTEX)
-        self.indices = np.ascontiguousarray(indices, dtype=np.uint32)
-        self.primitives = np.ascontiguousarray(primitives, dtype=layout.PRIMITIVE)
+    def __init__(self, vertices=None, indices=None, primitives=None):
+        def arr(a, dt):
+            return np.zeros(0, dtype=dt) if a is None else np.ascontiguousarray(a, dtype=dt).reshape(-1)
+        self.vertices = arr(vertices, layout.VERTEX)
+        self.indices = arr(indices, np.uint32)
+        self.primitives = arr(primitives, layout.PRIMITIVE)
+
+    def parse_gltf(self, path, share_geometry=False):
+        """Scene::parse_gltf (scene.rs:108-217): appends a glTF 2.0 file's mesh primitives and
+        one primitive per mesh primitive of each mesh node.  Raises NbError (NB_ERR_FORMAT with
+        the reference's MeshError text) and leaves the scene unchanged on a rejected file.
+
+        share_geometry: mesh primitives that name the same accessors share one copy of the
+        geometry and one index range (use with build_scene(share_blas=True))."""
+        h = ctypes.c_void_p()
+        check(lib.nb_host_scene_create(ctypes.byref(h)))
+        try:
+            check(lib.nb_host_scene_parse_gltf(h, str(path).encode(), 1 if share_geometry else 0))
+            nv, ni, nprim = ctypes.c_uint32(), ctypes.c_uint32(), ctypes.c_uint32()
+            check(lib.nb_host_scene_counts(h, ctypes.byref(nv), ctypes.byref(ni), ctypes.byref(nprim)))
+            v = np.zeros(nv.value, dtype=layout.VERTEX)
+            i = np.zeros(ni.value, dtype=np.uint32)
+            pr = np.zeros(nprim.value, dtype=layout.PRIMITIVE)
+            check(lib.nb_host_scene_copy(h, ptr(v), ptr(i), ptr(pr)))
+        finally:
+            lib.nb_host_scene_destroy(h)
+        # scene.rs:135-136: indices and index ranges continue after what the scene already holds
+        i += np.uint32(len(self.vertices))
+        pr["index_start"] += np.uint32(len(self.indices))
+        pr["index_end"] += np.uint32(len(self.indices))
+        self.vertices = np.concatenate([self.vertices, v])
+        self.indices = np.concatenate([self.indices, i])
+        self.primitives = np.concatenate([self.primitives, pr])
+        return self
 
 
 def build_scene(scene, share_blas=False):
