@@ -173,6 +173,27 @@ nb_status nb_target_read(nb_target *target, float *rgba);         /* waits for p
 nb_status nb_target_write(nb_target *target, const float *rgba);
 nb_status nb_target_clear(nb_target *target);
 
+/* ---- display and image files ------------------------------------------------------------
+ * nb_display stands for the RGBA16F surface (src/common/wgpu_context.rs:62) that BlitRenderer
+ * draws the target onto (src/pathtracing/blit_renderer.rs, blit.wgsl:27-38): a fullscreen
+ * triangle sampling the target bilinearly with clamp-to-edge addressing (texture.rs:64-74);
+ * texel weights carry 8 fractional bits.  Target row 0 is the bottom of the view; display row 0
+ * is the top of the surface.  nb_display_blit is asynchronous on the context stream;
+ * nb_display_read waits and copies width * height * 4 half floats. */
+typedef struct nb_display nb_display;
+nb_status nb_display_create(nb_context *ctx, uint32_t width, uint32_t height, nb_display **out);
+nb_status nb_display_destroy(nb_display *display);
+nb_status nb_display_blit(nb_display *display, const nb_target *target);
+nb_status nb_display_read(nb_display *display, uint16_t *rgba16f);
+
+/* Files of an RGBA32F image in target row order (as nb_target_read returns it): Portable Float
+ * Map (RGB, little-endian, rows bottom to top) and OpenEXR 2 (single-part scanline, no
+ * compression, channels A B G R as FLOAT, or HALF with NB_EXR_HALF; scanline 0 at the top). */
+#define NB_EXR_HALF 1u
+nb_status nb_image_write_pfm(const char *path, const float *rgba, uint32_t width, uint32_t height);
+nb_status nb_image_write_exr(const char *path, const float *rgba, uint32_t width, uint32_t height,
+                             uint32_t flags);
+
 /* ---- rendering (Pathtracer::dispatch, pathtracer.rs:225-239 -> pathtracing.wgsl:main) ----
  * nb_render: one sample per pixel of frame globals->sample with weight globals->weight,
  * color = mix(previous, sample, weight) (pathtracing.wgsl:247-260).

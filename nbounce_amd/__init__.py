@@ -3,8 +3,8 @@
 The compute path is libnbounce.so (HIP kernels + C ABI, include/nbounce.h); this package is its
 Python host mirror.  Importing it requires the built library.
 """
-from .pathtracer import (CameraController, Context, EnvMap, Globals, NbError, Pathtracer, Scene, SceneBuffers,
-                         build_scene, sobol_table)
+from .pathtracer import (BlitRenderer, CameraController, Context, EnvMap, Globals, NbError, Pathtracer, Scene,
+                         SceneBuffers, build_scene, sobol_table, write_exr, write_pfm)
 
-__all__ = ["CameraController", "Context", "EnvMap", "Globals", "NbError", "Pathtracer", "Scene", "SceneBuffers",
-           "build_scene", "sobol_table"]
+__all__ = ["BlitRenderer", "CameraController", "Context", "EnvMap", "Globals", "NbError", "Pathtracer", "Scene",
+           "SceneBuffers", "build_scene", "sobol_table", "write_exr", "write_pfm"]

@@ -44,6 +44,12 @@ SIGNATURES = {
     "nb_target_read": [_p, _p],
     "nb_target_write": [_p, _p],
     "nb_target_clear": [_p],
+    "nb_display_create": [_p, _u32, _u32, _c.POINTER(_p)],
+    "nb_display_destroy": [_p],
+    "nb_display_blit": [_p, _p],
+    "nb_display_read": [_p, _p],
+    "nb_image_write_pfm": [_c.c_char_p, _p, _u32, _u32],
+    "nb_image_write_exr": [_c.c_char_p, _p, _u32, _u32, _u32],
     "nb_render": [_p, _p, _p, _p, _p, _p],
     "nb_render_frames": [_p, _p, _p, _p, _p, _u32, _p],
     "nb_last_render_ms": [_p, _ptr(_f32)],
@@ -108,11 +114,13 @@ def check(status):
 
 
 def ptr(a):
-    """Address of a C-contiguous numpy array (or None for a null pointer)."""
+    """Address of a C-contiguous numpy array (or None for a null pointer).  The returned
+    c_void_p holds a reference to the array, so `f(ptr(make_array()))` cannot pass the address
+    of an array that was freed before the call."""
     if a is None:
         return None
     assert a.flags["C_CONTIGUOUS"], "arrays crossing the ABI must be C-contiguous"
-    return a.ctypes.data
+    return a.ctypes.data_as(_c.c_void_p)
 
 
 def struct_bytes(dtype, **fields):

@@ -299,6 +299,7 @@ RenderParams make_params(nb_context *ctx, const nb_scene *scene, nb_target *t, c
     p.n_tiles = (t->width / 8) * (t->height / 8);
     p.tiles_per_xcd = (p.n_tiles + 7) / 8;
     p.hits_out = nullptr;
+    p.work = ctx->work;
     return p;
 }
 
@@ -336,6 +337,8 @@ nb_status nb_context_create(int device, nb_context **out) {
     if (e == hipSuccess) e = hipEventCreate(&c->ev_region_stop);
     if (e == hipSuccess) e = hipMalloc(&c->counters, 8 * sizeof(unsigned long long));
     if (e == hipSuccess) e = hipMemset(c->counters, 0, 8 * sizeof(unsigned long long));
+    if (e == hipSuccess) e = hipMalloc(&c->work, 16 * sizeof(uint32_t));
+    if (e == hipSuccess) e = hipMemset(c->work, 0, 16 * sizeof(uint32_t));
     if (e != hipSuccess) {
         c->closing = true;
         release_context(c);
@@ -350,6 +353,7 @@ static void release_context(nb_context *ctx) {
     if (ctx->stream) hipStreamSynchronize(ctx->stream);
     hipFree(ctx->sobol);
     hipFree(ctx->counters);
+    hipFree(ctx->work);
     if (ctx->ev_start) hipEventDestroy(ctx->ev_start);
     if (ctx->ev_stop) hipEventDestroy(ctx->ev_stop);
     if (ctx->ev_region_start) hipEventDestroy(ctx->ev_region_start);

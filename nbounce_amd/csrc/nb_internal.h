@@ -80,6 +80,8 @@ struct RenderParams {
     unsigned long long *counters;  // rays, n_aabb, n_tri, n_inst, samples
     uint32_t tiles_x, n_tiles, tiles_per_xcd;
     uint32_t *hits_out;      // primary-hit debug output (5 words per pixel) or null
+    uint32_t *work;          // 8 per-XCD tile counters of the persistent kernel, zeroed per launch
+    uint32_t regen_min_idle; // the persistent kernel refills idle lanes once at least this many are idle
 };
 
 // ---- serving (serve.hip, server.cpp) ----------------------------------------------------------
@@ -144,6 +146,7 @@ struct nb_context {
     float4 *sobol = nullptr;
     uint32_t sobol_entries = 0, sobol_stride = 17;
     unsigned long long *counters = nullptr;
+    uint32_t *work = nullptr;       // tile counters of the persistent render kernel
     // Scenes, targets and environments keep their context alive: nb_context_destroy only marks
     // it, and the last child's destroy releases it.
     uint32_t children = 0;

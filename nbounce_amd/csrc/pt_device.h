@@ -234,6 +234,110 @@ __device__ __forceinline__ void trace(const DevScene &S, const Ray &world, Hit &
     }
 }
 
+// Resumable form of trace(): the same traversal, one step (an inner node, a TLAS leaf's instance
+// entry or a BLAS leaf's triangles, then the pop) per trace_step call, so that a wave can run
+// the traversal of some lanes while other lanes shade their finished hits.  trace_begin then
+// trace_step until it returns true selects exactly the hit trace() selects.
+struct TraceState {
+    Ray ray;                    // the ray in the space of the tree being walked
+    Hit hit;
+    uint32_t cur, blas_base, inst_cur, inst_end;
+    bool have;
+};
+
+template <int K>
+__device__ __forceinline__ void trace_begin(const DevScene &S, const Ray &world, TraceState &ts, Stack<K> &st) {
+    ts.hit.t = kNoHit;
+    ts.hit.tri = 0; ts.hit.inst = 0; ts.hit.u = 0.0f; ts.hit.v = 0.0f;
+    ts.ray = world;
+    st.sp = 0; st.lo = 0;
+    ts.blas_base = kNoRecord; ts.inst_cur = 0; ts.inst_end = 0;
+    ts.cur = S.tlas_root;
+    ts.have = S.tlas_root != kNoRecord;
+}
+
+// (W2L * (o, 1)).xyz and mat3(W2L) * d of instance ts.inst_cur, raytracing_sw.wgsl:203-205
+__device__ __forceinline__ void enter_instance(const DevScene &S, const Ray &world, TraceState &ts, uint32_t sp,
+                                               LaneCounters &c) {
+    const float4 *rec = S.inst + 4 * static_cast<size_t>(ts.inst_cur);
+    float4 r0 = rec[0], r1 = rec[1], r2 = rec[2];
+    uint32_t root = __float_as_uint(rec[3].x);
+    ts.ray.o = f3(((r0.x * world.o.x + r0.y * world.o.y) + r0.z * world.o.z) + r0.w * 1.0f,
+                  ((r1.x * world.o.x + r1.y * world.o.y) + r1.z * world.o.z) + r1.w * 1.0f,
+                  ((r2.x * world.o.x + r2.y * world.o.y) + r2.z * world.o.z) + r2.w * 1.0f);
+    ts.ray.d = f3((r0.x * world.d.x + r0.y * world.d.y) + r0.z * world.d.z,
+                  (r1.x * world.d.x + r1.y * world.d.y) + r1.z * world.d.z,
+                  (r2.x * world.d.x + r2.y * world.d.y) + r2.z * world.d.z);
+    ts.ray.inv = rcp3(ts.ray.d);
+    ++c.inst;
+    ts.blas_base = sp;
+    ts.cur = root;
+    ts.have = true;
+}
+
+// One traversal step; true when the closest hit of `world` is final in ts.hit.
+template <int K>
+__device__ __forceinline__ bool trace_step(const DevScene &S, const Ray &world, TraceState &ts, Stack<K> &st,
+                                           LaneCounters &c) {
+    if (ts.have) {
+        if (!(ts.cur & kLeafBit)) {
+            const float4 *rec = S.pairs + 4 * static_cast<size_t>(ts.cur);
+            float4 q0 = rec[0], q1 = rec[1], q2 = rec[2];
+            uint4 q3 = *reinterpret_cast<const uint4 *>(rec + 3);
+            float tl = slab(ts.ray, q0.x, q0.y, q0.z, q0.w, q1.x, q1.y);
+            float tr = slab(ts.ray, q1.z, q1.w, q2.x, q2.y, q2.z, q2.w);
+            c.aabb += q3.z ? 1u : 2u;
+            // ties: the right child is the near one (raytracing_sw.wgsl:229-234)
+            bool left_near = tl < tr;
+            uint32_t nref = left_near ? q3.x : q3.y, fref = left_near ? q3.y : q3.x;
+            float nt = left_near ? tl : tr, ft = left_near ? tr : tl;
+            if (ft < ts.hit.t) st.push(fref, ft);
+            if (nt < ts.hit.t) { ts.cur = nref; return false; }
+            ts.have = false;
+        } else if (ts.blas_base == kNoRecord) {
+            // TLAS leaf: instances [start, start + count), each searched in order
+            ts.inst_cur = ts.cur & kLeafStartMask;
+            ts.inst_end = ts.inst_cur + leaf_count(ts.cur, S.inst_leaf_count);
+            enter_instance(S, world, ts, st.sp, c);
+            return false;
+        } else {
+            const uint32_t start = ts.cur & kLeafStartMask, end = start + leaf_count(ts.cur, S.tri_leaf_count);
+            for (uint32_t j = start; j < end; ++j) {
+                const float4 *tr = S.tris + 3 * static_cast<size_t>(j);
+                float4 a = tr[0], b = tr[1], e = tr[2];
+                ++c.tri;
+                // Möller–Trumbore with back-face culling, raytracing_sw.wgsl:51-78
+                F3 e1 = xyz(b), e2 = xyz(e);
+                F3 h = cross(ts.ray.d, e2);
+                float det = dot(e1, h);
+                if (det < kEps) continue;
+                float inv_det = 1.0f / det;
+                F3 s = sub(ts.ray.o, xyz(a));
+                float u = inv_det * dot(s, h);
+                if (u < 0.0f || u > 1.0f) continue;
+                F3 q = cross(s, e1);
+                float v = inv_det * dot(ts.ray.d, q);
+                if (v < 0.0f || u + v > 1.0f) continue;
+                float t = inv_det * dot(e2, q);
+                if (t < kBias) continue;
+                if (t < ts.hit.t) { ts.hit.t = t; ts.hit.u = u; ts.hit.v = v; ts.hit.tri = j; ts.hit.inst = ts.inst_cur; }
+            }
+            ts.have = false;
+        }
+    }
+    // pop the next entry
+    if (ts.blas_base != kNoRecord && st.sp == ts.blas_base) {
+        if (++ts.inst_cur < ts.inst_end) { enter_instance(S, world, ts, st.sp, c); return false; }
+        ts.blas_base = kNoRecord;
+        ts.ray = world;
+    }
+    if (st.sp == 0) return true;
+    float t;
+    st.pop(ts.cur, t);
+    ts.have = t < ts.hit.t;   // skip entries farther than the current hit
+    return false;
+}
+
 // Bilinear cube lookup at LOD 0 with Vulkan face selection; faces carry a one-texel border
 // holding the seamless-edge neighbours, so the four taps never leave the padded face.
 __device__ F3 env_sample(const DevEnv &E, F3 d) {

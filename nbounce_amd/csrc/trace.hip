@@ -17,6 +17,7 @@
 #include "pt_device.h"
 
 #include <cstdlib>
+#include <cstring>
 
 namespace nb {
 namespace {
@@ -83,6 +84,165 @@ __global__ __launch_bounds__(64) void render_kernel(RenderParams p) {
     flush_counters(p, c, p.n_frames);
 }
 
+// Next tile for a wave on XCD `xcd`: its own band of tiles first, then the other XCDs' bands
+// (work stealing at the tail).  Returns n_tiles when every band is exhausted.  Wave-uniform.
+__device__ __forceinline__ uint32_t grab_tile(const RenderParams &p, uint32_t xcd) {
+    uint32_t t = p.n_tiles;
+    if (threadIdx.x == 0) {
+        for (uint32_t k = 0; k < 8u; ++k) {
+            const uint32_t b = (xcd + k) & 7u;
+            const uint32_t lo = b * p.tiles_per_xcd;
+            if (lo >= p.n_tiles) continue;
+            const uint32_t size = min(p.tiles_per_xcd, p.n_tiles - lo);
+            if (__hip_atomic_load(p.work + b, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= size) continue;
+            const uint32_t i = atomicAdd(p.work + b, 1u);
+            if (i < size) { t = lo + i; break; }
+        }
+    }
+    return __shfl(t, 0, 64);
+}
+
+// Persistent form of render_kernel with ray-level regeneration.  A fixed grid of wave64 workers
+// claims 8x8 tiles from per-XCD counters.  The traversal of each lane is resumable
+// (trace_step), and the wave alternates two phases:
+//   traverse: every lane with a ray in flight takes one traversal step, repeatedly, until enough
+//             lanes (min_service) have a finished hit or no work, or fewer lanes than that are
+//             still traversing and some are waiting;
+//   service:  lanes with a finished hit shade it (next ray, or path end and accumulation) and
+//             idle lanes take the next pixels of the wave's tile stream (ballot + prefix count),
+//             then start their traversal.
+// So lanes are not held idle while the longest traversal or path of the wave finishes.  A lane
+// keeps its pixel for all n_frames frames, in order: every sample and the mix chain are those of
+// render_kernel, and the image does not depend on which lane or wave renders a pixel.
+template <int K>
+__global__ __launch_bounds__(64) void render_pt_kernel(RenderParams p) {
+    __shared__ uint32_t s_ref[K * 64];
+    __shared__ float s_dist[K * 64];
+    const uint32_t lane = threadIdx.x;
+    const uint32_t xcd = blockIdx.x & 7u;
+    const uint32_t min_service = p.regen_min_idle;
+    Stack<K> st;
+    st.ref = (lds_u32 *)(s_ref + lane);
+    st.dist = (lds_f32 *)(s_dist + lane);
+    LaneCounters c{0, 0, 0, 0, 0};
+    PathConst pc;
+    pc.scene = p.scene;
+    pc.env = p.env;
+    pc.sobol = SobolView{p.sobol, p.sobol_entries, p.sobol_stride};
+    pc.bounces = p.bounces;
+    pc.contribution_factor = p.contribution_factor;
+
+    // wave-uniform tile stream: pixels [pos, 64) of tile `cur` come next, then tile `nxt`
+    uint32_t cur = grab_tile(p, xcd), pos = 0;
+    uint32_t nxt = cur < p.n_tiles ? grab_tile(p, xcd) : p.n_tiles;
+
+    enum : uint32_t { kIdle = 0, kTracing = 1, kReady = 2 };
+    uint32_t mode = kIdle;
+    uint32_t x = 0, y = 0, f = 0, bounce = 0, frames_done = 0;
+    Ray ray;                 // the path's current ray (world space)
+    TraceState ts;
+    F3 thr = f3(1.0f, 1.0f, 1.0f), col = f3(0.0f, 0.0f, 0.0f);
+    float w = 0.0f;
+    for (;;) {
+        bool start = false;      // this lane begins a new ray in this service phase
+        // ---- service: shade finished hits
+        if (mode == kReady) {
+            const Hit &h = ts.hit;
+            ++c.rays;
+            F3 out = f3(0.0f, 0.0f, 0.0f);
+            StepResult r;
+            if (h.t == kNoHit) {
+                out = mul(thr, env_sample(pc.env, ray.d));
+                r = kDone;
+            } else {
+                ++c.hits;
+                Vertex dummy;
+                r = shade<false>(pc, bounce, h, ray, thr, out, dummy);
+                if (r == kContinue && bounce == p.bounces) { r = kDone; out = f3(0.0f, 0.0f, 0.0f); }
+            }
+            if (r == kContinue) {
+                ++bounce;
+                mode = kTracing;
+                start = true;
+            } else {
+                // mix(color, s, weight) = color * (1 - weight) + s * weight
+                col = f3(col.x * (1.0f - w) + out.x * w, col.y * (1.0f - w) + out.y * w, col.z * (1.0f - w) + out.z * w);
+                bounce = 0;
+                ++frames_done;
+                if (++f == p.n_frames) {
+                    p.target[static_cast<size_t>(y) * p.width + x] = make_float4(col.x, col.y, col.z, 1.0f);
+                    mode = kIdle;
+                } else {
+                    w = 1.0f / static_cast<float>(p.first_sample + f);
+                    mode = kTracing;       // next frame of the same pixel: camera ray below
+                    start = true;
+                }
+            }
+        }
+        // ---- service: idle lanes take the next pixels of the stream, in lane order
+        const unsigned long long idle = __ballot(mode == kIdle);
+        if (idle != 0ull && cur < p.n_tiles) {
+            const uint32_t n_idle = static_cast<uint32_t>(__popcll(idle));
+            const uint32_t avail = 64u - pos;
+            if (mode == kIdle) {
+                const uint32_t rank = static_cast<uint32_t>(__popcll(idle & ((1ull << lane) - 1ull)));
+                uint32_t tile = p.n_tiles, k = 0;
+                if (rank < avail) { tile = cur; k = pos + rank; }
+                else if (nxt < p.n_tiles) { tile = nxt; k = rank - avail; }
+                if (tile < p.n_tiles) {
+                    x = (tile % p.tiles_x) * 8u + (k & 7u);
+                    y = (tile / p.tiles_x) * 8u + (k >> 3);
+                    mode = kTracing;
+                    start = true;
+                    f = 0;
+                    bounce = 0;
+                    pc.shift = hash4f(x, y, x, y);
+                    col = f3(0.0f, 0.0f, 0.0f);
+                    // first frame: the caller's weight (nb_render) or 1/sample (nb_render_frames)
+                    w = !p.weights_from_sample ? p.first_weight : 1.0f / static_cast<float>(p.first_sample);
+                    if (w > 0.0f) {
+                        const float4 prev = p.target[static_cast<size_t>(y) * p.width + x];
+                        col = f3(prev.x, prev.y, prev.z);
+                    }
+                }
+            }
+            if (n_idle < avail) {
+                pos += n_idle;
+            } else {
+                cur = nxt;
+                pos = n_idle - avail;   // < 64: pixels of the new current tile already taken
+                nxt = cur < p.n_tiles ? grab_tile(p, xcd) : p.n_tiles;
+            }
+        }
+        // ---- start the traversal of new rays (camera rays at bounce 0)
+        if (start) {
+            if (bounce == 0) {
+                pc.sample = p.first_sample + f;
+                const float4 jit = sobol_fetch(pc.sobol, pc.sample * p.sobol_stride, pc.shift);
+                ray = camera_ray(p.c2w, p.width, p.height, x, y, jit.x, jit.y);
+                thr = f3(1.0f, 1.0f, 1.0f);
+            }
+            trace_begin<K>(p.scene, ray, ts, st);
+        }
+        // every lane idle after the refill: the stream is exhausted
+        if (__ballot(mode != kIdle) == 0ull) break;
+        // ---- traverse
+        const bool more_pixels = cur < p.n_tiles;
+        for (;;) {
+            if (mode == kTracing && trace_step<K>(p.scene, ray, ts, st, c)) mode = kReady;
+            const uint32_t n_trace = static_cast<uint32_t>(__popcll(__ballot(mode == kTracing)));
+            const uint32_t n_ready = static_cast<uint32_t>(__popcll(__ballot(mode == kReady)));
+            const uint32_t n_wait = n_ready + (more_pixels ? 64u - n_trace - n_ready : 0u);
+            if (n_trace == 0 || n_wait >= min_service || (n_trace < min_service && n_wait > 0)) break;
+        }
+    }
+    flush_counters(p, c, frames_done);
+}
+
+__global__ void reset_work_kernel(uint32_t *work) {
+    if (threadIdx.x < 8) work[threadIdx.x] = 0u;
+}
+
 template <int K>
 __global__ __launch_bounds__(64) void primary_hits_kernel(RenderParams p) {
     __shared__ uint32_t s_ref[K * 64];
@@ -144,8 +304,65 @@ template <int K> struct RenderKernel { static constexpr auto fn = render_kernel<
 template <int K> struct HitsKernel { static constexpr auto fn = primary_hits_kernel<K>; };
 }  // namespace
 
+// Workgroups of the persistent kernel: 32 one-wave workgroups per CU (the most a CU holds);
+// the hardware starts what fits and the rest start as residents finish, find the tile counters
+// exhausted and exit.  NB_REGEN_WAVES_PER_CU overrides the 32.  A multiple of 8 (XCDs).
+static uint32_t regen_grid() {
+    static uint32_t grid = [] {
+        int dev = 0, cus = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+            return 0u;
+        const char *e = std::getenv("NB_REGEN_WAVES_PER_CU");
+        const int per_cu = e ? std::atoi(e) : 32;
+        const uint32_t g = static_cast<uint32_t>(cus) * static_cast<uint32_t>(per_cu < 1 ? 1 : per_cu) / 8u * 8u;
+        return g < 8u ? 8u : g;
+    }();
+    return grid;
+}
+
+// NB_REGEN_MIN_IDLE=n (1..64): a wave leaves its traversal phase to shade and refill once at
+// least n lanes wait (finished hit, or idle while pixels remain).
+static uint32_t regen_min_idle() {
+    static const uint32_t v = [] {
+        const char *e = std::getenv("NB_REGEN_MIN_IDLE");
+        const int n = e ? std::atoi(e) : 16;
+        return static_cast<uint32_t>(n < 1 ? 1 : (n > 64 ? 64 : n));
+    }();
+    return v;
+}
+
+template <int K>
+static nb_status launch_regen(const RenderParams &p_in, hipStream_t s) {
+    RenderParams p = p_in;
+    p.regen_min_idle = regen_min_idle();
+    const uint32_t grid = regen_grid();
+    if (grid == 0) return set_error(NB_ERR_HIP, "render: device attribute query failed");
+    hipLaunchKernelGGL(reset_work_kernel, dim3(1), dim3(64), 0, s, p.work);
+    hipLaunchKernelGGL(render_pt_kernel<K>, dim3(grid), dim3(64), 0, s, p);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return set_error(NB_ERR_HIP, "render launch: %s", hipGetErrorString(e));
+    return NB_OK;
+}
+
+// NB_RENDER=tile selects the one-tile-per-wave kernel (no regeneration) for comparisons.
+static bool use_tile_kernel() {
+    static const bool tile = [] {
+        const char *e = std::getenv("NB_RENDER");
+        return e && std::strcmp(e, "tile") == 0;
+    }();
+    return tile;
+}
+
 nb_status launch_render(const RenderParams &p, void *stream) {
-    return launch_with_k<RenderKernel>(p, stream, "render");
+    if (use_tile_kernel() || !p.work) return launch_with_k<RenderKernel>(p, stream, "render");
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    switch (lds_entries()) {
+    case 4: return launch_regen<4>(p, s);
+    case 8: return launch_regen<8>(p, s);
+    case 32: return launch_regen<32>(p, s);
+    default: return launch_regen<16>(p, s);
+    }
 }
 
 nb_status launch_primary_hits(const RenderParams &p, void *stream) {

@@ -13,7 +13,7 @@ from . import layout
 from ._lib import NbError, check, lib, ptr
 
 __all__ = ["Context", "Scene", "SceneBuffers", "CameraController", "EnvMap", "Globals", "Pathtracer",
-           "build_scene", "sobol_table", "NbError"]
+           "BlitRenderer", "build_scene", "sobol_table", "write_exr", "write_pfm", "NbError"]
 
 
 class Context:
@@ -350,3 +350,53 @@ class Pathtracer:
         out = np.zeros((h, w, 5), dtype=np.uint32)
         check(lib.nb_primary_hits(self.ctx.handle, self.scene.handle, ptr(self._camera_data), w, h, sample, ptr(out)))
         return out
+
+
+class BlitRenderer:
+    """Draws the accumulation target onto an RGBA16F surface (BlitRenderer,
+    src/pathtracing/blit_renderer.rs; blit.wgsl:27-38; surface format wgpu_context.rs:62):
+    bilinear, clamp-to-edge, surface row 0 at the top.  `render(pathtracer)` enqueues the blit of
+    its output texture; `surface()` waits and returns the (height, width, 4) float16 image."""
+
+    def __init__(self, ctx, width, height):
+        h = ctypes.c_void_p()
+        check(lib.nb_display_create(ctx.handle, int(width), int(height), ctypes.byref(h)))
+        self.handle, self.ctx, self.width, self.height = h, ctx, int(width), int(height)
+
+    def render(self, pathtracer):
+        check(lib.nb_display_blit(self.handle, pathtracer.output.handle))
+
+    def surface(self):
+        out = np.zeros((self.height, self.width, 4), dtype=np.float16)
+        check(lib.nb_display_read(self.handle, ptr(out)))
+        return out
+
+    def close(self):
+        if getattr(self, "handle", None):
+            lib.nb_display_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def _image_arg(rgba):
+    rgba = np.ascontiguousarray(rgba, dtype=np.float32)
+    if rgba.ndim != 3 or rgba.shape[2] != 4:
+        raise ValueError(f"expected a (height, width, 4) RGBA image, got shape {rgba.shape}")
+    return rgba
+
+
+def write_pfm(path, rgba):
+    """Portable Float Map (RGB) of an image in target row order (output_texture())."""
+    rgba = _image_arg(rgba)
+    check(lib.nb_image_write_pfm(str(path).encode(), ptr(rgba), rgba.shape[1], rgba.shape[0]))
+
+
+def write_exr(path, rgba, half=False):
+    """Uncompressed scanline OpenEXR of an image in target row order (output_texture())."""
+    rgba = _image_arg(rgba)
+    check(lib.nb_image_write_exr(str(path).encode(), ptr(rgba), rgba.shape[1], rgba.shape[0], 1 if half else 0))
