@@ -263,6 +263,7 @@ def main():
     ap.add_argument("--qps", type=float, default=20.0)
     ap.add_argument("--requests", type=int, default=200)
     ap.add_argument("--max-paths", type=int, default=1 << 23)
+    ap.add_argument("--count-in-timed", action="store_true", help="keep the kernel counters on in the timed pass")
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="also run one device copy of this many MiB (counter calibration under rocprofv3)")
     args = ap.parse_args()
@@ -305,8 +306,16 @@ def main():
         calibration_copy(args.calibrate_copy_mib)
     step_block(args.warmup)
     ctx.synchronize()
+    # The counters (the reference's n_aabb / n_tri debug variables plus rays, instance entries,
+    # hits) are taken over an untimed pass of the same frames; the timed pass renders the same
+    # frames with counting off.  Paths depend only on (pixel, sample, scene): the counts match.
     pt.invalidate()
     ctx.reset_counters()
+    step_block(args.steps)
+    ctx.synchronize()
+    c = ctx.counters()
+    pt.invalidate()
+    ctx.enable_counters(args.count_in_timed)
     ctx.synchronize()
     t0 = time.perf_counter()
     ctx.timer_start()
@@ -314,7 +323,7 @@ def main():
     dev_ms = ctx.timer_stop()
     ctx.synchronize()
     wall = time.perf_counter() - t0
-    c = ctx.counters()
+    ctx.enable_counters(True)
     launches = args.steps if F == 1 else math.ceil(args.steps / F)
     rays = c["rays"]
     value = rays / wall / 1e6

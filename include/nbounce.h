@@ -214,6 +214,9 @@ nb_status nb_stream_timer_stop(nb_context *ctx, float *ms);
 /* ---- counters / debug ------------------------------------------------------------------- */
 nb_status nb_counters_get(nb_context *ctx, nb_counters *out);
 nb_status nb_counters_reset(nb_context *ctx);
+/* Counting is on by default; off, the render kernels skip the counter atomics (the image is
+ * the same).  The reference's n_aabb / n_tri are debug variables of the shader. */
+nb_status nb_counters_enable(nb_context *ctx, uint32_t on);
 /* The closest hit of every primary ray of frame `sample` (the commented-out debug view of
  * pathtracing.wgsl:262 over intersect_TLAS, raytracing_sw.wgsl:176): out[5*p..] =
  * {instance, i0, i1, i2, float bits of dist}, instance = 0xFFFFFFFF on a miss. */

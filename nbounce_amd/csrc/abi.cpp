@@ -7,6 +7,7 @@
 
 #include <cmath>
 #include <cstdarg>
+#include <cstdlib>
 #include <cstdio>
 #include <cstring>
 #include <map>
@@ -125,42 +126,61 @@ nb_status relayout(const nb_bvh_node *blas, uint32_t n_blas, const nb_bvh_node *
     for (uint32_t j = 0; j < n_inst; ++j)
         if (inst[j].node >= n_blas) return set_error(NB_ERR_BAD_LAYOUT, "instance %u: BLAS root %u out of range", j, inst[j].node);
 
-    // record ids: BLAS inner nodes, BLAS root records, TLAS inner nodes, TLAS root record
+    // Record ids in depth-first preorder of each tree (left child first), each tree preceded by
+    // its root record: a record's left child is the next record, so descents stay in one cache
+    // line half of the time.  BLAS trees in order of first use by an instance, then the TLAS.
+    // Nodes no instance reaches get no record.
     std::vector<uint32_t> pid_b(n_blas, kNoRecord), pid_t(n_tlas, kNoRecord);
-    uint32_t next = 0;
-    for (uint32_t i = 0; i < n_blas; ++i) if (blas[i].end == 0) pid_b[i] = next++;
+    struct Rec { uint8_t kind; uint32_t node; };     // 0 BLAS inner, 1 BLAS root, 2 TLAS inner, 3 TLAS root
+    std::vector<Rec> recs;
     std::map<uint32_t, uint32_t> root_rec;
-    for (uint32_t j = 0; j < n_inst; ++j) root_rec.emplace(inst[j].node, 0u);
-    for (auto &kv : root_rec) kv.second = next++;
-    for (uint32_t i = 0; i < n_tlas; ++i) if (tlas[i].end == 0) pid_t[i] = next++;
-    if (n_tlas > 0) R.tlas_root = next++;
+    std::vector<uint32_t> stack;
+    auto dfs = [&](const nb_bvh_node *nodes, uint32_t root, std::vector<uint32_t> &pid, uint8_t kind) {
+        if (nodes[root].end != 0) return;
+        stack.assign(1, root);
+        while (!stack.empty()) {
+            const uint32_t i = stack.back();
+            stack.pop_back();
+            if (pid[i] != kNoRecord) continue;   // a node reached twice keeps its first record
+            pid[i] = static_cast<uint32_t>(recs.size());
+            recs.push_back({kind, i});
+            const uint32_t l = nodes[i].start, r = l + 1;
+            if (nodes[r].end == 0) stack.push_back(r);
+            if (nodes[l].end == 0) stack.push_back(l);
+        }
+    };
+    for (uint32_t j = 0; j < n_inst; ++j) {
+        const uint32_t root = inst[j].node;
+        if (root_rec.count(root)) continue;
+        root_rec[root] = static_cast<uint32_t>(recs.size());
+        recs.push_back({1, root});
+        if (pid_b[root] == kNoRecord) dfs(blas, root, pid_b, 0);
+    }
+    if (n_tlas > 0) {
+        R.tlas_root = static_cast<uint32_t>(recs.size());
+        recs.push_back({3, 0u});
+        dfs(tlas, 0, pid_t, 2);
+    }
 
     R.tri_leaf.assign(n_tris ? n_tris : 1, 0u);
     R.inst_leaf.assign(n_inst ? n_inst : 1, 0u);
-    R.pairs.reserve(4 * static_cast<size_t>(next));
-    for (uint32_t i = 0; i < n_blas; ++i) {
-        if (blas[i].end != 0) continue;
-        uint32_t l = blas[i].start, r = l + 1, lr, rr;
-        NB_TRY(child_ref(blas[l], pid_b, l, R.tri_leaf, &lr));
-        NB_TRY(child_ref(blas[r], pid_b, r, R.tri_leaf, &rr));
-        push_pair(R.pairs, blas[l], lr, &blas[r], rr, 0u);
-    }
-    for (auto &kv : root_rec) {
-        uint32_t ref;
-        NB_TRY(child_ref(blas[kv.first], pid_b, kv.first, R.tri_leaf, &ref));
-        push_pair(R.pairs, blas[kv.first], ref, nullptr, 0u, 1u);
-    }
-    for (uint32_t i = 0; i < n_tlas; ++i) {
-        if (tlas[i].end != 0) continue;
-        uint32_t l = tlas[i].start, r = l + 1, lr, rr;
-        NB_TRY(child_ref(tlas[l], pid_t, l, R.inst_leaf, &lr));
-        NB_TRY(child_ref(tlas[r], pid_t, r, R.inst_leaf, &rr));
-        push_pair(R.pairs, tlas[l], lr, &tlas[r], rr, 0u);
-    }
-    if (n_tlas > 0) {
-        uint32_t ref;
-        NB_TRY(child_ref(tlas[0], pid_t, 0, R.inst_leaf, &ref));
-        push_pair(R.pairs, tlas[0], ref, nullptr, 0u, 1u);
+    R.pairs.reserve(4 * recs.size());
+    for (const Rec &rc : recs) {
+        const bool tl = rc.kind >= 2;
+        const nb_bvh_node *nodes = tl ? tlas : blas;
+        std::vector<uint32_t> &pid = tl ? pid_t : pid_b;
+        std::vector<uint32_t> &side = tl ? R.inst_leaf : R.tri_leaf;
+        if (rc.kind == 1 || rc.kind == 3) {
+            uint32_t ref;
+            NB_TRY(child_ref(nodes[rc.node], pid, rc.node, side, &ref));
+            push_pair(R.pairs, nodes[rc.node], ref, nullptr, 0u, 1u);
+        } else {
+            const uint32_t l = nodes[rc.node].start, r = l + 1;
+            uint32_t lr, rr;
+            NB_TRY(child_ref(nodes[l], pid, l, side, &lr));
+            NB_TRY(child_ref(nodes[r], pid, r, side, &rr));
+            push_pair(R.pairs, nodes[l], lr, &nodes[r], rr, 0u);
+        }
     }
 
     R.tris.resize(3 * static_cast<size_t>(n_tris));
@@ -294,7 +314,7 @@ RenderParams make_params(nb_context *ctx, const nb_scene *scene, nb_target *t, c
     p.weights_from_sample = 0;
     p.bounces = g->bounces;
     p.contribution_factor = g->contribution_factor;
-    p.counters = ctx->counters;
+    p.counters = ctx->counting ? ctx->counters : nullptr;
     p.tiles_x = t->width / 8;
     p.n_tiles = (t->width / 8) * (t->height / 8);
     p.tiles_per_xcd = (p.n_tiles + 7) / 8;
@@ -303,9 +323,79 @@ RenderParams make_params(nb_context *ctx, const nb_scene *scene, nb_target *t, c
     return p;
 }
 
-nb_status timed_launch(nb_context *ctx, const RenderParams &p, nb_status (*launch)(const RenderParams &, void *)) {
+// NB_RENDER selects the render kernels: "wavefront" (default: trace and shade kernels per bounce
+// level), "pt" (persistent megakernel) or "tile" (one 8x8 tile per wave).
+enum class RenderMode { kWavefront, kPersistent, kTile };
+RenderMode render_mode() {
+    static const RenderMode m = [] {
+        const char *e = std::getenv("NB_RENDER");
+        if (e && std::strcmp(e, "pt") == 0) return RenderMode::kPersistent;
+        if (e && std::strcmp(e, "tile") == 0) return RenderMode::kTile;
+        return RenderMode::kWavefront;
+    }();
+    return m;
+}
+
+nb_status wavefront_buffers(nb_context *ctx, size_t n) {
+    if (n <= ctx->wf_capacity) return NB_OK;
+    NB_HIP(hipStreamSynchronize(ctx->stream));
+    void **bufs[] = {reinterpret_cast<void **>(&ctx->wf_queue[0]), reinterpret_cast<void **>(&ctx->wf_queue[1]),
+                     reinterpret_cast<void **>(&ctx->wf_path), reinterpret_cast<void **>(&ctx->wf_hit),
+                     reinterpret_cast<void **>(&ctx->wf_hit_inst)};
+    const size_t sizes[] = {32 * n, 32 * n, 16 * n, 16 * n, 4 * n};
+    for (void **b : bufs) { hipFree(*b); *b = nullptr; }
+    ctx->wf_capacity = 0;
+    for (int i = 0; i < 5; ++i) {
+        hipError_t e = hipMalloc(bufs[i], sizes[i]);
+        if (e != hipSuccess) {
+            for (void **b : bufs) { hipFree(*b); *b = nullptr; }
+            return set_error(NB_ERR_OUT_OF_MEMORY, "wavefront buffers for %zu rays: %s", n, hipGetErrorString(e));
+        }
+    }
+    if (!ctx->wf_counts) NB_HIP(hipMalloc(&ctx->wf_counts, 2 * (kWfMaxLevels + 1) * sizeof(uint32_t)));
+    ctx->wf_capacity = n;
+    return NB_OK;
+}
+
+// The frames of `p` with the selected kernels (every mode produces the same image).
+nb_status launch_frames(nb_context *ctx, const RenderParams &p, void *stream) {
+    if (render_mode() != RenderMode::kWavefront || p.bounces + 1u > kWfMaxLevels || p.hits_out)
+        return launch_render(p, stream);
+    const size_t n = static_cast<size_t>(p.width) * p.height;
+    NB_TRY(wavefront_buffers(ctx, n));
+    WfParams w{};
+    w.scene = p.scene;
+    w.env = p.env;
+    w.target = p.target;
+    w.width = p.width;
+    w.height = p.height;
+    w.tiles_x = p.tiles_x;
+    std::memcpy(w.c2w, p.c2w, sizeof(w.c2w));
+    w.sobol = p.sobol;
+    w.sobol_entries = p.sobol_entries;
+    w.sobol_stride = p.sobol_stride;
+    w.bounces = p.bounces;
+    w.contribution_factor = p.contribution_factor;
+    w.queue[0] = ctx->wf_queue[0];
+    w.queue[1] = ctx->wf_queue[1];
+    w.path = ctx->wf_path;
+    w.hit = ctx->wf_hit;
+    w.hit_inst = ctx->wf_hit_inst;
+    w.counts = ctx->wf_counts;
+    w.counters = p.counters;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    for (uint32_t f = 0; f < p.n_frames; ++f) {
+        w.sample = p.first_sample + f;
+        w.weight = (f == 0 && !p.weights_from_sample) ? p.first_weight : 1.0f / static_cast<float>(w.sample);
+        NB_TRY(wavefront_frame(w, p.bounces + 1u, s));
+    }
+    return NB_OK;
+}
+
+
+nb_status timed_frames(nb_context *ctx, const RenderParams &p) {
     NB_HIP(hipEventRecord(ctx->ev_start, ctx->stream));
-    NB_TRY(launch(p, ctx->stream));
+    NB_TRY(launch_frames(ctx, p, ctx->stream));
     NB_HIP(hipEventRecord(ctx->ev_stop, ctx->stream));
     ctx->timed = true;
     return NB_OK;
@@ -335,8 +425,8 @@ nb_status nb_context_create(int device, nb_context **out) {
     if (e == hipSuccess) e = hipEventCreate(&c->ev_stop);
     if (e == hipSuccess) e = hipEventCreate(&c->ev_region_start);
     if (e == hipSuccess) e = hipEventCreate(&c->ev_region_stop);
-    if (e == hipSuccess) e = hipMalloc(&c->counters, 8 * sizeof(unsigned long long));
-    if (e == hipSuccess) e = hipMemset(c->counters, 0, 8 * sizeof(unsigned long long));
+    if (e == hipSuccess) e = hipMalloc(&c->counters, kCounterBytes);
+    if (e == hipSuccess) e = hipMemset(c->counters, 0, kCounterBytes);
     if (e == hipSuccess) e = hipMalloc(&c->work, 16 * sizeof(uint32_t));
     if (e == hipSuccess) e = hipMemset(c->work, 0, 16 * sizeof(uint32_t));
     if (e != hipSuccess) {
@@ -354,6 +444,12 @@ static void release_context(nb_context *ctx) {
     hipFree(ctx->sobol);
     hipFree(ctx->counters);
     hipFree(ctx->work);
+    hipFree(ctx->wf_queue[0]);
+    hipFree(ctx->wf_queue[1]);
+    hipFree(ctx->wf_path);
+    hipFree(ctx->wf_hit);
+    hipFree(ctx->wf_hit_inst);
+    hipFree(ctx->wf_counts);
     if (ctx->ev_start) hipEventDestroy(ctx->ev_start);
     if (ctx->ev_stop) hipEventDestroy(ctx->ev_stop);
     if (ctx->ev_region_start) hipEventDestroy(ctx->ev_region_start);
@@ -599,7 +695,7 @@ nb_status nb_render(nb_context *ctx, const nb_scene *scene, nb_target *target, c
     NB_TRY(check_render_args(ctx, scene, target, camera, globals, env));
     NB_TRY(use_device(ctx));
     RenderParams p = make_params(ctx, scene, target, camera, globals, env);
-    return timed_launch(ctx, p, launch_render);
+    return timed_frames(ctx, p);
 }
 
 nb_status nb_render_frames(nb_context *ctx, const nb_scene *scene, nb_target *target, const nb_camera *camera,
@@ -612,7 +708,7 @@ nb_status nb_render_frames(nb_context *ctx, const nb_scene *scene, nb_target *ta
     RenderParams p = make_params(ctx, scene, target, camera, first, env);
     p.n_frames = n_frames;
     p.weights_from_sample = 1;
-    return timed_launch(ctx, p, launch_render);
+    return timed_frames(ctx, p);
 }
 
 nb_status nb_last_render_ms(nb_context *ctx, float *ms) {
@@ -647,16 +743,24 @@ nb_status nb_counters_get(nb_context *ctx, nb_counters *out) {
     if (!ctx || !out) return set_error(NB_ERR_INVALID_ARGUMENT, "null argument");
     NB_TRY(use_device(ctx));
     NB_HIP(hipStreamSynchronize(ctx->stream));
-    unsigned long long v[8];
+    unsigned long long v[kCounterSlots * kCounterWords], sum[kCounterWords] = {};
     NB_HIP(hipMemcpy(v, ctx->counters, sizeof(v), hipMemcpyDeviceToHost));
-    out->rays = v[0]; out->n_aabb = v[1]; out->n_tri = v[2]; out->n_inst = v[3]; out->samples = v[4]; out->hits = v[5];
+    for (uint32_t i = 0; i < kCounterSlots * kCounterWords; ++i) sum[i % kCounterWords] += v[i];
+    out->rays = sum[0]; out->n_aabb = sum[1]; out->n_tri = sum[2]; out->n_inst = sum[3]; out->samples = sum[4];
+    out->hits = sum[5];
+    return NB_OK;
+}
+
+nb_status nb_counters_enable(nb_context *ctx, uint32_t on) {
+    if (!ctx) return set_error(NB_ERR_INVALID_ARGUMENT, "null context");
+    ctx->counting = on != 0;
     return NB_OK;
 }
 
 nb_status nb_counters_reset(nb_context *ctx) {
     if (!ctx) return set_error(NB_ERR_INVALID_ARGUMENT, "null context");
     NB_TRY(use_device(ctx));
-    NB_HIP(hipMemsetAsync(ctx->counters, 0, 8 * sizeof(unsigned long long), ctx->stream));
+    NB_HIP(hipMemsetAsync(ctx->counters, 0, kCounterBytes, ctx->stream));
     return NB_OK;
 }
 

@@ -322,7 +322,7 @@ nb_status nb_render_cached(nb_context *ctx, const nb_scene *scene, nb_target *ta
     f.t_raw = c->raw;
     f.t_target = c->target;
     f.t_count = c->counts + 1;
-    f.counters = ctx->counters;
+    f.counters = ctx->counting ? ctx->counters : nullptr;
     f.tiles_x = target->width / 8;
     f.n_tiles = (target->width / 8) * (target->height / 8);
     f.tiles_per_xcd = (f.n_tiles + 7) / 8;

@@ -53,18 +53,6 @@ __device__ __forceinline__ uint32_t wave_prefix_alloc(bool want, uint32_t *count
     return total;
 }
 
-__device__ __forceinline__ void flush(unsigned long long *counters, const LaneCounters &c, uint32_t samples) {
-    auto sum = [](uint32_t v) { for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64); return v; };
-    uint32_t r = sum(c.rays), a = sum(c.aabb), t = sum(c.tri), i = sum(c.inst), s = sum(samples), h = sum(c.hits);
-    if ((threadIdx.x & 63u) == 0 && counters) {
-        atomicAdd(counters + 0, static_cast<unsigned long long>(r));
-        atomicAdd(counters + 1, static_cast<unsigned long long>(a));
-        atomicAdd(counters + 2, static_cast<unsigned long long>(t));
-        atomicAdd(counters + 3, static_cast<unsigned long long>(i));
-        atomicAdd(counters + 4, static_cast<unsigned long long>(s));
-        atomicAdd(counters + 5, static_cast<unsigned long long>(h));
-    }
-}
 
 // One sample per pixel: the path runs as in sample_rendering_eq until the first shading point
 // after a bounce whose lobe was not a near mirror (GGX alpha >= mirror_alpha) or until bounce
@@ -78,7 +66,9 @@ __global__ __launch_bounds__(64) void cache_query_kernel(CacheFrame p) {
     const uint32_t lane = threadIdx.x;
     const uint32_t x = (tile % p.tiles_x) * 8u + (lane & 7u), y = (tile / p.tiles_x) * 8u + (lane >> 3);
     const uint32_t pixel = y * p.width + x;
+    SpillArray spill;
     Stack<kCK> st;
+    st.spill = spill;
     st.ref = (lds_u32 *)(s_ref + lane);
     st.dist = (lds_f32 *)(s_dist + lane);
     LaneCounters c{0, 0, 0, 0, 0};
@@ -117,7 +107,7 @@ __global__ __launch_bounds__(64) void cache_query_kernel(CacheFrame p) {
         store_raw(p.q_raw + static_cast<size_t>(slot) * kMlpRaw, raw);
         reinterpret_cast<float4 *>(p.q_thr)[slot] = make_float4(thr.x, thr.y, thr.z, __uint_as_float(pixel));
     }
-    flush(p.counters, c, 1);
+    flush_counters(p.counters, c, 1);
 }
 
 // Full unbiased paths at a spread of pixels; every shading point (up to three per path) becomes
@@ -127,7 +117,9 @@ __global__ __launch_bounds__(64) void cache_train_paths_kernel(CacheFrame p) {
     __shared__ uint32_t s_ref[kCK * 64];
     __shared__ float s_dist[kCK * 64];
     const uint32_t lane = threadIdx.x;
+    SpillArray spill;
     Stack<kCK> st;
+    st.spill = spill;
     st.ref = (lds_u32 *)(s_ref + lane);
     st.dist = (lds_f32 *)(s_dist + lane);
     LaneCounters c{0, 0, 0, 0, 0};
@@ -186,7 +178,7 @@ __global__ __launch_bounds__(64) void cache_train_paths_kernel(CacheFrame p) {
             }
         }
     }
-    flush(p.counters, c, 0);
+    flush_counters(p.counters, c, 0);
 }
 
 // direct[pixel] += throughput * max(cache radiance, 0) for every query.

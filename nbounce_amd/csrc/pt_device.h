@@ -97,19 +97,41 @@ __device__ __forceinline__ float slab(const Ray &r, float lx, float ly, float lz
 
 struct LaneCounters { uint32_t rays, aabb, tri, inst, hits; };
 
+// Adds a wave's counters (and `samples` finished samples) to the counter slot of the XCD the
+// wave runs on (workgroup b runs on XCD b % 8), so the atomics of the XCDs do not contend for
+// one address.  Waves with nothing to add issue no atomics.  Every lane of the wave calls this.
+__device__ __forceinline__ void flush_counters(unsigned long long *counters, const LaneCounters &c,
+                                               uint32_t samples) {
+    auto sum = [](uint32_t v) { for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64); return v; };
+    const uint32_t r = sum(c.rays), a = sum(c.aabb), t = sum(c.tri), i = sum(c.inst), s = sum(samples), h = sum(c.hits);
+    if ((threadIdx.x & 63u) == 0 && counters && (r | a | t | i | s | h)) {
+        unsigned long long *slot = counters + kCounterWords * (blockIdx.x % kCounterSlots);
+        if (r) atomicAdd(slot + 0, static_cast<unsigned long long>(r));
+        if (a) atomicAdd(slot + 1, static_cast<unsigned long long>(a));
+        if (t) atomicAdd(slot + 2, static_cast<unsigned long long>(t));
+        if (i) atomicAdd(slot + 3, static_cast<unsigned long long>(i));
+        if (s) atomicAdd(slot + 4, static_cast<unsigned long long>(s));
+        if (h) atomicAdd(slot + 5, static_cast<unsigned long long>(h));
+    }
+}
+
 struct Hit { float t, u, v; uint32_t tri, inst; };
 
 // LDS (address space 3) views, so stack accesses compile to ds_read / ds_write.
 typedef __attribute__((address_space(3))) uint32_t lds_u32;
 typedef __attribute__((address_space(3))) float lds_f32;
 
-// Traversal stack: top K entries per lane in LDS, older entries in private memory.
+// Traversal stack: top K entries per lane in LDS, older entries in a private array owned by the
+// kernel (declared beside the Stack, so the Stack's own fields stay in registers: a struct that
+// holds a dynamically indexed array is placed in scratch memory as a whole).
+typedef uint2 SpillArray[kStackEntries];
+
 template <int K>
 struct Stack {
     lds_u32 *ref;   // slot s of this lane at ref[s * 64]
     lds_f32 *dist;
+    uint2 *spill;   // kStackEntries entries
     uint32_t sp, lo;
-    uint2 spill[kStackEntries];
 
     __device__ __forceinline__ void push(uint32_t r, float t) {
         if (sp >= kStackEntries) return;  // unreachable for trees of depth <= 32 (checked at upload)

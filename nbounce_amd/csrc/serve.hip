@@ -63,11 +63,6 @@ __global__ __launch_bounds__(256) void serve_generate_kernel(const ServeStep *st
     }
 }
 
-__device__ __forceinline__ uint32_t wave_sum_u32(uint32_t v) {
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-    return v;
-}
-
 __global__ __launch_bounds__(64) void serve_bounce_kernel(const ServeStep *step, uint32_t level) {
     __shared__ uint32_t s_ref[kServeK * 64];
     __shared__ float s_dist[kServeK * 64];
@@ -76,7 +71,9 @@ __global__ __launch_bounds__(64) void serve_bounce_kernel(const ServeStep *step,
     const uint32_t n_in = S.level[2 * level];
     const float4 *in = S.queue[level & 1u];
     float4 *out = S.queue[(level + 1u) & 1u];
+    SpillArray spill;
     Stack<kServeK> st;
+    st.spill = spill;
     st.ref = (lds_u32 *)(s_ref + lane);
     st.dist = (lds_f32 *)(s_dist + lane);
     LaneCounters c{0, 0, 0, 0, 0};
@@ -137,16 +134,7 @@ __global__ __launch_bounds__(64) void serve_bounce_kernel(const ServeStep *step,
             q[0] = q0; q[1] = q1; q[2] = q2;
         }
     }
-    const uint32_t r = wave_sum_u32(c.rays), a = wave_sum_u32(c.aabb), t = wave_sum_u32(c.tri);
-    const uint32_t in_ = wave_sum_u32(c.inst), s = wave_sum_u32(samples_done), h = wave_sum_u32(c.hits);
-    if (lane == 0 && S.counters) {
-        atomicAdd(S.counters + 0, static_cast<unsigned long long>(r));
-        atomicAdd(S.counters + 1, static_cast<unsigned long long>(a));
-        atomicAdd(S.counters + 2, static_cast<unsigned long long>(t));
-        atomicAdd(S.counters + 3, static_cast<unsigned long long>(in_));
-        atomicAdd(S.counters + 4, static_cast<unsigned long long>(s));
-        atomicAdd(S.counters + 5, static_cast<unsigned long long>(h));
-    }
+    flush_counters(S.counters, c, samples_done);
 }
 
 // color = mix(color, s, 1/sample) for the step's samples of each pixel, in sample order.

@@ -182,8 +182,8 @@ nb_status nb_server_create(nb_context *ctx, uint64_t budget, uint32_t max_paths,
     if (e == hipSuccess) e = hipMalloc(&s->queue[1], qbytes);
     if (e == hipSuccess) e = hipMalloc(&s->results, static_cast<size_t>(max_paths) * 16);
     if (e == hipSuccess) e = hipMalloc(&s->level, 2 * (kServeMaxBounces + 2) * sizeof(uint32_t));
-    if (e == hipSuccess) e = hipMalloc(&s->counters, 8 * sizeof(unsigned long long));
-    if (e == hipSuccess) e = hipMemset(s->counters, 0, 8 * sizeof(unsigned long long));
+    if (e == hipSuccess) e = hipMalloc(&s->counters, kCounterBytes);
+    if (e == hipSuccess) e = hipMemset(s->counters, 0, kCounterBytes);
     if (e == hipSuccess) e = hipMalloc(&s->d_step, sizeof(ServeStep));
     if (e == hipSuccess) e = hipMalloc(&s->d_passes, kMaxSlots * sizeof(ServePass));
     if (e == hipSuccess) e = hipMalloc(&s->d_requests, kMaxSlots * sizeof(ServeRequestDev));
@@ -370,11 +370,13 @@ nb_status nb_server_stats_get(nb_server *s, nb_server_stats *out) {
     if (!s || !out) return set_error(NB_ERR_INVALID_ARGUMENT, "null argument");
     SRV_HIP(hipSetDevice(s->ctx->device));
     SRV_HIP(hipStreamSynchronize(s->stream));
-    unsigned long long v[8];
+    unsigned long long v[kCounterSlots * kCounterWords], sum[kCounterWords] = {};
     SRV_HIP(hipMemcpy(v, s->counters, sizeof(v), hipMemcpyDeviceToHost));
+    for (uint32_t i = 0; i < kCounterSlots * kCounterWords; ++i) sum[i % kCounterWords] += v[i];
     std::memset(out, 0, sizeof(*out));
     out->steps = s->steps;
-    out->rays = v[0]; out->n_aabb = v[1]; out->n_tri = v[2]; out->n_inst = v[3]; out->samples = v[4]; out->hits = v[5];
+    out->rays = sum[0]; out->n_aabb = sum[1]; out->n_tri = sum[2]; out->n_inst = sum[3]; out->samples = sum[4];
+    out->hits = sum[5];
     out->bytes_in_use = s->in_use;
     out->bytes_budget = s->budget;
     out->active = static_cast<uint32_t>(s->active.size());

@@ -23,23 +23,6 @@ namespace nb {
 namespace {
 using namespace dev;
 
-__device__ __forceinline__ uint32_t wave_sum(uint32_t v) {
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-    return v;
-}
-
-__device__ __forceinline__ void flush_counters(const RenderParams &p, const LaneCounters &c, uint32_t samples) {
-    uint32_t r = wave_sum(c.rays), a = wave_sum(c.aabb), t = wave_sum(c.tri), i = wave_sum(c.inst), s = wave_sum(samples);
-    uint32_t h = wave_sum(c.hits);
-    if (threadIdx.x == 0 && p.counters) {
-        atomicAdd(p.counters + 0, static_cast<unsigned long long>(r));
-        atomicAdd(p.counters + 1, static_cast<unsigned long long>(a));
-        atomicAdd(p.counters + 2, static_cast<unsigned long long>(t));
-        atomicAdd(p.counters + 3, static_cast<unsigned long long>(i));
-        atomicAdd(p.counters + 4, static_cast<unsigned long long>(s));
-        atomicAdd(p.counters + 5, static_cast<unsigned long long>(h));
-    }
-}
 
 // XCD-aware tile order: workgroups are dealt round-robin over the 8 XCDs, so block b runs on
 // XCD b % 8; give each XCD a contiguous run of tiles so neighbouring (coherent) tiles share an L2.
@@ -56,7 +39,9 @@ __global__ __launch_bounds__(64) void render_kernel(RenderParams p) {
     if (tile >= p.n_tiles) return;
     const uint32_t lane = threadIdx.x;
     const uint32_t x = (tile % p.tiles_x) * 8u + (lane & 7u), y = (tile / p.tiles_x) * 8u + (lane >> 3);
+    SpillArray spill;
     Stack<K> st;
+    st.spill = spill;
     st.ref = (lds_u32 *)(s_ref + lane);
     st.dist = (lds_f32 *)(s_dist + lane);
     LaneCounters c{0, 0, 0, 0, 0};
@@ -81,7 +66,7 @@ __global__ __launch_bounds__(64) void render_kernel(RenderParams p) {
         col = f3(col.x * (1.0f - w) + s.x * w, col.y * (1.0f - w) + s.y * w, col.z * (1.0f - w) + s.z * w);
     }
     *px = make_float4(col.x, col.y, col.z, 1.0f);
-    flush_counters(p, c, p.n_frames);
+    flush_counters(p.counters, c, p.n_frames);
 }
 
 // Next tile for a wave on XCD `xcd`: its own band of tiles first, then the other XCDs' bands
@@ -121,7 +106,9 @@ __global__ __launch_bounds__(64) void render_pt_kernel(RenderParams p) {
     const uint32_t lane = threadIdx.x;
     const uint32_t xcd = blockIdx.x & 7u;
     const uint32_t min_service = p.regen_min_idle;
+    SpillArray spill;
     Stack<K> st;
+    st.spill = spill;
     st.ref = (lds_u32 *)(s_ref + lane);
     st.dist = (lds_f32 *)(s_dist + lane);
     LaneCounters c{0, 0, 0, 0, 0};
@@ -236,7 +223,7 @@ __global__ __launch_bounds__(64) void render_pt_kernel(RenderParams p) {
             if (n_trace == 0 || n_wait >= min_service || (n_trace < min_service && n_wait > 0)) break;
         }
     }
-    flush_counters(p, c, frames_done);
+    flush_counters(p.counters, c, frames_done);
 }
 
 __global__ void reset_work_kernel(uint32_t *work) {
@@ -251,7 +238,9 @@ __global__ __launch_bounds__(64) void primary_hits_kernel(RenderParams p) {
     if (tile >= p.n_tiles) return;
     const uint32_t lane = threadIdx.x;
     const uint32_t x = (tile % p.tiles_x) * 8u + (lane & 7u), y = (tile / p.tiles_x) * 8u + (lane >> 3);
+    SpillArray spill;
     Stack<K> st;
+    st.spill = spill;
     st.ref = (lds_u32 *)(s_ref + lane);
     st.dist = (lds_f32 *)(s_dist + lane);
     LaneCounters c{0, 0, 0, 0, 0};

@@ -33,6 +33,9 @@ class Context:
         check(lib.nb_counters_get(self.handle, ptr(c)))
         return {k: int(c[k][0]) for k in layout.COUNTERS.names}
 
+    def enable_counters(self, on=True):
+        check(lib.nb_counters_enable(self.handle, 1 if on else 0))
+
     def reset_counters(self):
         check(lib.nb_counters_reset(self.handle))
 
