@@ -323,15 +323,16 @@ RenderParams make_params(nb_context *ctx, const nb_scene *scene, nb_target *t, c
     return p;
 }
 
-// NB_RENDER selects the render kernels: "wavefront" (default: trace and shade kernels per bounce
-// level), "pt" (persistent megakernel) or "tile" (one 8x8 tile per wave).
-enum class RenderMode { kWavefront, kPersistent, kTile };
+// NB_RENDER selects the render kernels: "pt" (default: the persistent megakernel), "wavefront"
+// (trace and shade kernels per bounce level) or "tile" (one 8x8 tile per wave).  Every choice
+// renders the same image.
+enum class RenderMode { kPersistent, kWavefront, kTile };
 RenderMode render_mode() {
     static const RenderMode m = [] {
         const char *e = std::getenv("NB_RENDER");
-        if (e && std::strcmp(e, "pt") == 0) return RenderMode::kPersistent;
+        if (e && std::strcmp(e, "wavefront") == 0) return RenderMode::kWavefront;
         if (e && std::strcmp(e, "tile") == 0) return RenderMode::kTile;
-        return RenderMode::kWavefront;
+        return RenderMode::kPersistent;
     }();
     return m;
 }

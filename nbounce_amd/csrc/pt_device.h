@@ -351,7 +351,10 @@ __device__ __forceinline__ bool trace_step(const DevScene &S, const Ray &world, 
     if (ts.blas_base != kNoRecord && st.sp == ts.blas_base) {
         if (++ts.inst_cur < ts.inst_end) { enter_instance(S, world, ts, st.sp, c); return false; }
         ts.blas_base = kNoRecord;
-        ts.ray = world;
+        // back in world space; 1/d is recomputed (the same bits) so callers need not keep it
+        ts.ray.o = world.o;
+        ts.ray.d = world.d;
+        ts.ray.inv = rcp3(world.d);
     }
     if (st.sp == 0) return true;
     float t;

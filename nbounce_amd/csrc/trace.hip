@@ -99,7 +99,7 @@ __device__ __forceinline__ uint32_t grab_tile(const RenderParams &p, uint32_t xc
 // So lanes are not held idle while the longest traversal or path of the wave finishes.  A lane
 // keeps its pixel for all n_frames frames, in order: every sample and the mix chain are those of
 // render_kernel, and the image does not depend on which lane or wave renders a pixel.
-template <int K>
+template <int K, bool kCount>
 __global__ __launch_bounds__(64) void render_pt_kernel(RenderParams p) {
     __shared__ uint32_t s_ref[K * 64];
     __shared__ float s_dist[K * 64];
@@ -144,6 +144,7 @@ __global__ __launch_bounds__(64) void render_pt_kernel(RenderParams p) {
             } else {
                 ++c.hits;
                 Vertex dummy;
+                pc.shift = hash4f(x, y, x, y);   // recomputed: not kept live across traversal
                 r = shade<false>(pc, bounce, h, ray, thr, out, dummy);
                 if (r == kContinue && bounce == p.bounces) { r = kDone; out = f3(0.0f, 0.0f, 0.0f); }
             }
@@ -183,7 +184,6 @@ __global__ __launch_bounds__(64) void render_pt_kernel(RenderParams p) {
                     start = true;
                     f = 0;
                     bounce = 0;
-                    pc.shift = hash4f(x, y, x, y);
                     col = f3(0.0f, 0.0f, 0.0f);
                     // first frame: the caller's weight (nb_render) or 1/sample (nb_render_frames)
                     w = !p.weights_from_sample ? p.first_weight : 1.0f / static_cast<float>(p.first_sample);
@@ -205,7 +205,7 @@ __global__ __launch_bounds__(64) void render_pt_kernel(RenderParams p) {
         if (start) {
             if (bounce == 0) {
                 pc.sample = p.first_sample + f;
-                const float4 jit = sobol_fetch(pc.sobol, pc.sample * p.sobol_stride, pc.shift);
+                const float4 jit = sobol_fetch(pc.sobol, pc.sample * p.sobol_stride, hash4f(x, y, x, y));
                 ray = camera_ray(p.c2w, p.width, p.height, x, y, jit.x, jit.y);
                 thr = f3(1.0f, 1.0f, 1.0f);
             }
@@ -223,7 +223,7 @@ __global__ __launch_bounds__(64) void render_pt_kernel(RenderParams p) {
             if (n_trace == 0 || n_wait >= min_service || (n_trace < min_service && n_wait > 0)) break;
         }
     }
-    flush_counters(p.counters, c, frames_done);
+    if (kCount) flush_counters(p.counters, c, frames_done);   // else the counts are dead code
 }
 
 __global__ void reset_work_kernel(uint32_t *work) {
@@ -328,7 +328,8 @@ static nb_status launch_regen(const RenderParams &p_in, hipStream_t s) {
     const uint32_t grid = regen_grid();
     if (grid == 0) return set_error(NB_ERR_HIP, "render: device attribute query failed");
     hipLaunchKernelGGL(reset_work_kernel, dim3(1), dim3(64), 0, s, p.work);
-    hipLaunchKernelGGL(render_pt_kernel<K>, dim3(grid), dim3(64), 0, s, p);
+    if (p.counters) hipLaunchKernelGGL((render_pt_kernel<K, true>), dim3(grid), dim3(64), 0, s, p);
+    else hipLaunchKernelGGL((render_pt_kernel<K, false>), dim3(grid), dim3(64), 0, s, p);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return set_error(NB_ERR_HIP, "render launch: %s", hipGetErrorString(e));
     return NB_OK;
