@@ -323,80 +323,9 @@ RenderParams make_params(nb_context *ctx, const nb_scene *scene, nb_target *t, c
     return p;
 }
 
-// NB_RENDER selects the render kernels: "pt" (default: the persistent megakernel), "wavefront"
-// (trace and shade kernels per bounce level) or "tile" (one 8x8 tile per wave).  Every choice
-// renders the same image.
-enum class RenderMode { kPersistent, kWavefront, kTile };
-RenderMode render_mode() {
-    static const RenderMode m = [] {
-        const char *e = std::getenv("NB_RENDER");
-        if (e && std::strcmp(e, "wavefront") == 0) return RenderMode::kWavefront;
-        if (e && std::strcmp(e, "tile") == 0) return RenderMode::kTile;
-        return RenderMode::kPersistent;
-    }();
-    return m;
-}
-
-nb_status wavefront_buffers(nb_context *ctx, size_t n) {
-    if (n <= ctx->wf_capacity) return NB_OK;
-    NB_HIP(hipStreamSynchronize(ctx->stream));
-    void **bufs[] = {reinterpret_cast<void **>(&ctx->wf_queue[0]), reinterpret_cast<void **>(&ctx->wf_queue[1]),
-                     reinterpret_cast<void **>(&ctx->wf_path), reinterpret_cast<void **>(&ctx->wf_hit),
-                     reinterpret_cast<void **>(&ctx->wf_hit_inst)};
-    const size_t sizes[] = {32 * n, 32 * n, 16 * n, 16 * n, 4 * n};
-    for (void **b : bufs) { hipFree(*b); *b = nullptr; }
-    ctx->wf_capacity = 0;
-    for (int i = 0; i < 5; ++i) {
-        hipError_t e = hipMalloc(bufs[i], sizes[i]);
-        if (e != hipSuccess) {
-            for (void **b : bufs) { hipFree(*b); *b = nullptr; }
-            return set_error(NB_ERR_OUT_OF_MEMORY, "wavefront buffers for %zu rays: %s", n, hipGetErrorString(e));
-        }
-    }
-    if (!ctx->wf_counts) NB_HIP(hipMalloc(&ctx->wf_counts, 2 * (kWfMaxLevels + 1) * sizeof(uint32_t)));
-    ctx->wf_capacity = n;
-    return NB_OK;
-}
-
-// The frames of `p` with the selected kernels (every mode produces the same image).
-nb_status launch_frames(nb_context *ctx, const RenderParams &p, void *stream) {
-    if (render_mode() != RenderMode::kWavefront || p.bounces + 1u > kWfMaxLevels || p.hits_out)
-        return launch_render(p, stream);
-    const size_t n = static_cast<size_t>(p.width) * p.height;
-    NB_TRY(wavefront_buffers(ctx, n));
-    WfParams w{};
-    w.scene = p.scene;
-    w.env = p.env;
-    w.target = p.target;
-    w.width = p.width;
-    w.height = p.height;
-    w.tiles_x = p.tiles_x;
-    std::memcpy(w.c2w, p.c2w, sizeof(w.c2w));
-    w.sobol = p.sobol;
-    w.sobol_entries = p.sobol_entries;
-    w.sobol_stride = p.sobol_stride;
-    w.bounces = p.bounces;
-    w.contribution_factor = p.contribution_factor;
-    w.queue[0] = ctx->wf_queue[0];
-    w.queue[1] = ctx->wf_queue[1];
-    w.path = ctx->wf_path;
-    w.hit = ctx->wf_hit;
-    w.hit_inst = ctx->wf_hit_inst;
-    w.counts = ctx->wf_counts;
-    w.counters = p.counters;
-    hipStream_t s = static_cast<hipStream_t>(stream);
-    for (uint32_t f = 0; f < p.n_frames; ++f) {
-        w.sample = p.first_sample + f;
-        w.weight = (f == 0 && !p.weights_from_sample) ? p.first_weight : 1.0f / static_cast<float>(w.sample);
-        NB_TRY(wavefront_frame(w, p.bounces + 1u, s));
-    }
-    return NB_OK;
-}
-
-
 nb_status timed_frames(nb_context *ctx, const RenderParams &p) {
     NB_HIP(hipEventRecord(ctx->ev_start, ctx->stream));
-    NB_TRY(launch_frames(ctx, p, ctx->stream));
+    NB_TRY(launch_render(p, ctx->stream));
     NB_HIP(hipEventRecord(ctx->ev_stop, ctx->stream));
     ctx->timed = true;
     return NB_OK;
@@ -445,12 +374,6 @@ static void release_context(nb_context *ctx) {
     hipFree(ctx->sobol);
     hipFree(ctx->counters);
     hipFree(ctx->work);
-    hipFree(ctx->wf_queue[0]);
-    hipFree(ctx->wf_queue[1]);
-    hipFree(ctx->wf_path);
-    hipFree(ctx->wf_hit);
-    hipFree(ctx->wf_hit_inst);
-    hipFree(ctx->wf_counts);
     if (ctx->ev_start) hipEventDestroy(ctx->ev_start);
     if (ctx->ev_stop) hipEventDestroy(ctx->ev_stop);
     if (ctx->ev_region_start) hipEventDestroy(ctx->ev_region_start);

@@ -87,37 +87,8 @@ struct RenderParams {
     uint32_t *hits_out;      // primary-hit debug output (5 words per pixel) or null
     uint32_t *work;          // 8 per-XCD tile counters of the persistent kernel, zeroed per launch
     uint32_t regen_min_idle; // the persistent kernel refills idle lanes once at least this many are idle
+    uint32_t inner_quorum;   // inner-node steps run alone while more lanes than this have one
 };
-
-// ---- wavefront renderer (wavefront.hip) -------------------------------------------------------
-constexpr uint32_t kWfMaxLevels = 256;   // bounce levels per frame (bounces + 1)
-
-// One frame of the wavefront renderer.  Ray queues hold 32-B records (o.xyz, path id)
-// (d.xyz, 0); path ids index `path` (throughput rgb, bounce) and map to pixels in 8x8 tile
-// order; `hit` / `hit_inst` are indexed by queue position; counts[2l] = rays queued at level l,
-// counts[2l+1] = rays claimed by the trace kernel.
-struct WfParams {
-    DevScene scene;
-    DevEnv env;
-    float4 *target;
-    uint32_t width, height, tiles_x;
-    float c2w[16];
-    const float4 *sobol;
-    uint32_t sobol_entries, sobol_stride;
-    uint32_t sample;
-    float weight;
-    uint32_t bounces;
-    float contribution_factor;
-    float4 *queue[2];
-    float4 *path;
-    uint4 *hit;
-    uint32_t *hit_inst;
-    uint32_t *counts;
-    unsigned long long *counters;
-    uint32_t min_fetch;      // idle lanes that trigger a refill in the trace kernel
-};
-
-nb_status wavefront_frame(const WfParams &p, uint32_t levels, hipStream_t s);
 
 // ---- serving (serve.hip, server.cpp) ----------------------------------------------------------
 struct ServeSceneDev {
@@ -183,13 +154,6 @@ struct nb_context {
     unsigned long long *counters = nullptr;   // kCounterSlots x kCounterWords
     bool counting = true;                     // nb_counters_enable
     uint32_t *work = nullptr;       // tile counters of the persistent render kernel
-    // wavefront renderer buffers, grown to the largest target rendered
-    size_t wf_capacity = 0;
-    float4 *wf_queue[2] = {nullptr, nullptr};
-    float4 *wf_path = nullptr;
-    uint4 *wf_hit = nullptr;
-    uint32_t *wf_hit_inst = nullptr;
-    uint32_t *wf_counts = nullptr;
     // Scenes, targets and environments keep their context alive: nb_context_destroy only marks
     // it, and the last child's destroy releases it.
     uint32_t children = 0;

@@ -165,101 +165,16 @@ __device__ __forceinline__ uint32_t leaf_count(uint32_t ref, const uint32_t *sid
     return c != 0 ? c : side[ref & kLeafStartMask];
 }
 
-// Closest hit of `world` over the two-level BVH (intersect_TLAS / intersect_BLAS,
-// raytracing_sw.wgsl:176-319).
-template <int K>
-__device__ __forceinline__ void trace(const DevScene &S, const Ray &world, Hit &hit, Stack<K> &st, LaneCounters &c) {
-    hit.t = kNoHit;
-    hit.tri = 0; hit.inst = 0; hit.u = 0.0f; hit.v = 0.0f;
-    if (S.tlas_root == kNoRecord) return;
-    Ray ray = world;
-    st.sp = 0; st.lo = 0;
-    uint32_t blas_base = kNoRecord, inst_cur = 0, inst_end = 0;
-    uint32_t cur = S.tlas_root;
-    bool have = true;
-    for (;;) {
-        if (have) {
-            if (!(cur & kLeafBit)) {
-                const float4 *rec = S.pairs + 4 * static_cast<size_t>(cur);
-                float4 q0 = rec[0], q1 = rec[1], q2 = rec[2];
-                uint4 q3 = *reinterpret_cast<const uint4 *>(rec + 3);
-                float tl = slab(ray, q0.x, q0.y, q0.z, q0.w, q1.x, q1.y);
-                float tr = slab(ray, q1.z, q1.w, q2.x, q2.y, q2.z, q2.w);
-                c.aabb += q3.z ? 1u : 2u;
-                // ties: the right child is the near one (raytracing_sw.wgsl:229-234)
-                bool left_near = tl < tr;
-                uint32_t nref = left_near ? q3.x : q3.y, fref = left_near ? q3.y : q3.x;
-                float nt = left_near ? tl : tr, ft = left_near ? tr : tl;
-                if (ft < hit.t) st.push(fref, ft);
-                if (nt < hit.t) { cur = nref; continue; }
-                have = false;
-            } else if (blas_base == kNoRecord) {
-                // TLAS leaf: instances [start, start + count), each searched in order
-                inst_cur = cur & kLeafStartMask;
-                inst_end = inst_cur + leaf_count(cur, S.inst_leaf_count);
-                goto enter_instance;
-            } else {
-                const uint32_t start = cur & kLeafStartMask, end = start + leaf_count(cur, S.tri_leaf_count);
-                for (uint32_t j = start; j < end; ++j) {
-                    const float4 *tr = S.tris + 3 * static_cast<size_t>(j);
-                    float4 a = tr[0], b = tr[1], e = tr[2];
-                    ++c.tri;
-                    // Möller–Trumbore with back-face culling, raytracing_sw.wgsl:51-78
-                    F3 e1 = xyz(b), e2 = xyz(e);
-                    F3 h = cross(ray.d, e2);
-                    float det = dot(e1, h);
-                    if (det < kEps) continue;
-                    float inv_det = 1.0f / det;
-                    F3 s = sub(ray.o, xyz(a));
-                    float u = inv_det * dot(s, h);
-                    if (u < 0.0f || u > 1.0f) continue;
-                    F3 q = cross(s, e1);
-                    float v = inv_det * dot(ray.d, q);
-                    if (v < 0.0f || u + v > 1.0f) continue;
-                    float t = inv_det * dot(e2, q);
-                    if (t < kBias) continue;
-                    if (t < hit.t) { hit.t = t; hit.u = u; hit.v = v; hit.tri = j; hit.inst = inst_cur; }
-                }
-                have = false;
-            }
-        }
-        // pop the next entry
-        if (blas_base != kNoRecord && st.sp == blas_base) {
-            if (++inst_cur < inst_end) goto enter_instance;
-            blas_base = kNoRecord;
-            ray = world;
-        }
-        if (st.sp == 0) break;
-        {
-            float t;
-            st.pop(cur, t);
-            have = t < hit.t;   // skip entries farther than the current hit
-        }
-        continue;
-    enter_instance: {
-            const float4 *rec = S.inst + 4 * static_cast<size_t>(inst_cur);
-            float4 r0 = rec[0], r1 = rec[1], r2 = rec[2];
-            uint32_t root = __float_as_uint(rec[3].x);
-            // (W2L * (o, 1)).xyz and mat3(W2L) * d, raytracing_sw.wgsl:203-205
-            ray.o = f3(((r0.x * world.o.x + r0.y * world.o.y) + r0.z * world.o.z) + r0.w * 1.0f,
-                       ((r1.x * world.o.x + r1.y * world.o.y) + r1.z * world.o.z) + r1.w * 1.0f,
-                       ((r2.x * world.o.x + r2.y * world.o.y) + r2.z * world.o.z) + r2.w * 1.0f);
-            ray.d = f3((r0.x * world.d.x + r0.y * world.d.y) + r0.z * world.d.z,
-                       (r1.x * world.d.x + r1.y * world.d.y) + r1.z * world.d.z,
-                       (r2.x * world.d.x + r2.y * world.d.y) + r2.z * world.d.z);
-            ray.inv = rcp3(ray.d);
-            ++c.inst;
-            blas_base = st.sp;
-            cur = root;
-            have = true;
-        }
-    }
-}
-
-// Resumable form of trace(): the same traversal, one step (an inner node, a TLAS leaf's instance
+// Two-level BVH traversal in resumable form: one step (an inner node, a TLAS leaf's instance
 // entry or a BLAS leaf's triangles, then the pop) per trace_step call, so that a wave can run
-// the traversal of some lanes while other lanes shade their finished hits.  trace_begin then
-// trace_step until it returns true selects exactly the hit trace() selects.
+// the traversal of some lanes while other lanes shade their finished hits.
+//
+// TLAS and BLAS share one stack; entering an instance records the stack height at which its BLAS
+// ends.  Ties follow the reference: the right child is the near one when the entry distances
+// are equal, and a later triangle or instance replaces the hit only if strictly nearer.  The
+// current closest distance is handed into each BLAS (the reference starts every BLAS search
+// from infinity, raytracing_sw.wgsl:255): this prunes boxes and triangles that could not win,
+// so the selected hit is the same.
 struct TraceState {
     Ray ray;                    // the ray in the space of the tree being walked
     Hit hit;
@@ -297,25 +212,53 @@ __device__ __forceinline__ void enter_instance(const DevScene &S, const Ray &wor
     ts.have = true;
 }
 
+// The inner-node part of a traversal step at record ts.cur: test both children, push the far one
+// if it is nearer than the current hit, continue with the near one (have = true) or not.
+template <int K>
+__device__ __forceinline__ void visit_inner(const DevScene &S, TraceState &ts, Stack<K> &st, LaneCounters &c) {
+    const float4 *rec = S.pairs + 4 * static_cast<size_t>(ts.cur);
+    float4 q0 = rec[0], q1 = rec[1], q2 = rec[2];
+    uint4 q3 = *reinterpret_cast<const uint4 *>(rec + 3);
+    float tl = slab(ts.ray, q0.x, q0.y, q0.z, q0.w, q1.x, q1.y);
+    float tr = slab(ts.ray, q1.z, q1.w, q2.x, q2.y, q2.z, q2.w);
+    c.aabb += q3.z ? 1u : 2u;
+    // ties: the right child is the near one (raytracing_sw.wgsl:229-234)
+    bool left_near = tl < tr;
+    uint32_t nref = left_near ? q3.x : q3.y, fref = left_near ? q3.y : q3.x;
+    float nt = left_near ? tl : tr, ft = left_near ? tr : tl;
+    if (ft < ts.hit.t) st.push(fref, ft);
+    ts.have = nt < ts.hit.t;
+    if (ts.have) ts.cur = nref;
+}
+
+// True when the next action of a lane is an inner-node visit or a pop that neither leaves a
+// BLAS nor ends the trace: the cheap, common steps that inner_steps runs.
+template <int K>
+__device__ __forceinline__ bool inner_pending(const TraceState &ts, const Stack<K> &st) {
+    if (ts.have) return !(ts.cur & kLeafBit);
+    return st.sp != 0 && !(ts.blas_base != kNoRecord && st.sp == ts.blas_base);
+}
+
+// One inner-node visit or plain pop (requires inner_pending).
+template <int K>
+__device__ __forceinline__ void inner_step(const DevScene &S, TraceState &ts, Stack<K> &st, LaneCounters &c) {
+    if (ts.have) {
+        visit_inner<K>(S, ts, st, c);
+    } else {
+        float t;
+        st.pop(ts.cur, t);
+        ts.have = t < ts.hit.t;   // skip entries farther than the current hit
+    }
+}
+
 // One traversal step; true when the closest hit of `world` is final in ts.hit.
 template <int K>
 __device__ __forceinline__ bool trace_step(const DevScene &S, const Ray &world, TraceState &ts, Stack<K> &st,
                                            LaneCounters &c) {
     if (ts.have) {
         if (!(ts.cur & kLeafBit)) {
-            const float4 *rec = S.pairs + 4 * static_cast<size_t>(ts.cur);
-            float4 q0 = rec[0], q1 = rec[1], q2 = rec[2];
-            uint4 q3 = *reinterpret_cast<const uint4 *>(rec + 3);
-            float tl = slab(ts.ray, q0.x, q0.y, q0.z, q0.w, q1.x, q1.y);
-            float tr = slab(ts.ray, q1.z, q1.w, q2.x, q2.y, q2.z, q2.w);
-            c.aabb += q3.z ? 1u : 2u;
-            // ties: the right child is the near one (raytracing_sw.wgsl:229-234)
-            bool left_near = tl < tr;
-            uint32_t nref = left_near ? q3.x : q3.y, fref = left_near ? q3.y : q3.x;
-            float nt = left_near ? tl : tr, ft = left_near ? tr : tl;
-            if (ft < ts.hit.t) st.push(fref, ft);
-            if (nt < ts.hit.t) { ts.cur = nref; return false; }
-            ts.have = false;
+            visit_inner<K>(S, ts, st, c);
+            if (ts.have) return false;
         } else if (ts.blas_base == kNoRecord) {
             // TLAS leaf: instances [start, start + count), each searched in order
             ts.inst_cur = ts.cur & kLeafStartMask;
@@ -361,6 +304,16 @@ __device__ __forceinline__ bool trace_step(const DevScene &S, const Ray &world, 
     st.pop(ts.cur, t);
     ts.have = t < ts.hit.t;   // skip entries farther than the current hit
     return false;
+}
+
+// Closest hit of `world` over the two-level BVH (intersect_TLAS / intersect_BLAS,
+// raytracing_sw.wgsl:176-319).
+template <int K>
+__device__ __forceinline__ void trace(const DevScene &S, const Ray &world, Hit &hit, Stack<K> &st, LaneCounters &c) {
+    TraceState ts;
+    trace_begin<K>(S, world, ts, st);
+    while (!trace_step<K>(S, world, ts, st, c)) {}
+    hit = ts.hit;
 }
 
 // Bilinear cube lookup at LOD 0 with Vulkan face selection; faces carry a one-texel border
