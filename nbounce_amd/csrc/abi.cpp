@@ -99,18 +99,16 @@ nb_status child_ref(const nb_bvh_node &n, const std::vector<uint32_t> &pid, uint
     return NB_OK;
 }
 
-void push_pair(std::vector<float4> &pairs, const nb_bvh_node &l, uint32_t lref, const nb_bvh_node *r, uint32_t rref,
-               uint32_t is_root) {
+// One child-pair record (4 float4): boxes of l and r (r == nullptr: an all-NaN box) and refs.
+void make_pair(const nb_bvh_node &l, uint32_t lref, const nb_bvh_node *r, uint32_t rref, uint32_t is_root, float4 *q) {
     const float nan = std::nanf("");
     float rmin[3] = {nan, nan, nan}, rmax[3] = {nan, nan, nan};
     if (r) { std::memcpy(rmin, r->min, 12); std::memcpy(rmax, r->max, 12); }
-    pairs.push_back(make_float4(l.min[0], l.min[1], l.min[2], l.max[0]));
-    pairs.push_back(make_float4(l.max[1], l.max[2], rmin[0], rmin[1]));
-    pairs.push_back(make_float4(rmin[2], rmax[0], rmax[1], rmax[2]));
-    float4 q3;
+    q[0] = make_float4(l.min[0], l.min[1], l.min[2], l.max[0]);
+    q[1] = make_float4(l.max[1], l.max[2], rmin[0], rmin[1]);
+    q[2] = make_float4(rmin[2], rmax[0], rmax[1], rmax[2]);
     uint32_t w[4] = {lref, rref, is_root, 0u};
-    std::memcpy(&q3, w, 16);
-    pairs.push_back(q3);
+    std::memcpy(&q[3], w, 16);
 }
 
 nb_status relayout(const nb_bvh_node *blas, uint32_t n_blas, const nb_bvh_node *tlas, uint32_t n_tlas,
@@ -164,22 +162,24 @@ nb_status relayout(const nb_bvh_node *blas, uint32_t n_blas, const nb_bvh_node *
 
     R.tri_leaf.assign(n_tris ? n_tris : 1, 0u);
     R.inst_leaf.assign(n_inst ? n_inst : 1, 0u);
-    R.pairs.reserve(4 * recs.size());
-    for (const Rec &rc : recs) {
+    R.pairs.assign(kRecordFloat4 * recs.size(), make_float4(0.0f, 0.0f, 0.0f, 0.0f));
+    for (size_t k = 0; k < recs.size(); ++k) {
+        const Rec &rc = recs[k];
         const bool tl = rc.kind >= 2;
         const nb_bvh_node *nodes = tl ? tlas : blas;
         std::vector<uint32_t> &pid = tl ? pid_t : pid_b;
         std::vector<uint32_t> &side = tl ? R.inst_leaf : R.tri_leaf;
+        float4 *q = R.pairs.data() + kRecordFloat4 * k;
         if (rc.kind == 1 || rc.kind == 3) {
             uint32_t ref;
             NB_TRY(child_ref(nodes[rc.node], pid, rc.node, side, &ref));
-            push_pair(R.pairs, nodes[rc.node], ref, nullptr, 0u, 1u);
+            make_pair(nodes[rc.node], ref, nullptr, 0u, 1u, q);
         } else {
             const uint32_t l = nodes[rc.node].start, r = l + 1;
             uint32_t lr, rr;
             NB_TRY(child_ref(nodes[l], pid, l, side, &lr));
             NB_TRY(child_ref(nodes[r], pid, r, side, &rr));
-            push_pair(R.pairs, nodes[l], lr, &nodes[r], rr, 0u);
+            make_pair(nodes[l], lr, &nodes[r], rr, 0u, q);
         }
     }
 
@@ -471,7 +471,7 @@ nb_status nb_scene_create(nb_context *ctx, const nb_bvh_node *blas, uint32_t n_b
     s->dev.tri_leaf_count = reinterpret_cast<const uint32_t *>(b + off[5]);
     s->dev.inst_leaf_count = reinterpret_cast<const uint32_t *>(b + off[6]);
     s->dev.tlas_root = R.tlas_root;
-    s->dev.n_pairs = static_cast<uint32_t>(R.pairs.size() / 4);
+    s->dev.n_pairs = static_cast<uint32_t>(R.pairs.size() / kRecordFloat4);
     ++ctx->children;
     *out = s;
     return NB_OK;

@@ -59,8 +59,7 @@ __device__ __forceinline__ uint32_t wave_prefix_alloc(bool want, uint32_t *count
 // max_cache_bounce; there it stops and queues a cache query with its throughput.  Radiance met
 // before (sky, emitters) goes to `direct`.
 __global__ __launch_bounds__(64) void cache_query_kernel(CacheFrame p) {
-    __shared__ uint32_t s_ref[kCK * 64];
-    __shared__ float s_dist[kCK * 64];
+    __shared__ uint2 s_stack[kCK * 64];
     const uint32_t tile = (blockIdx.x & 7u) * p.tiles_per_xcd + (blockIdx.x >> 3);
     if (tile >= p.n_tiles) return;
     const uint32_t lane = threadIdx.x;
@@ -68,9 +67,7 @@ __global__ __launch_bounds__(64) void cache_query_kernel(CacheFrame p) {
     const uint32_t pixel = y * p.width + x;
     SpillArray spill;
     Stack<kCK> st;
-    st.spill = spill;
-    st.ref = (lds_u32 *)(s_ref + lane);
-    st.dist = (lds_f32 *)(s_dist + lane);
+    st.init(s_stack, lane, spill);
     LaneCounters c{0, 0, 0, 0, 0};
     PathConst pc;
     pc.scene = p.scene;
@@ -114,14 +111,11 @@ __global__ __launch_bounds__(64) void cache_query_kernel(CacheFrame p) {
 // a training record whose target is the path's final radiance divided by the throughput that
 // arrived at that point, i.e. the one-sample estimate of the point's outgoing radiance.
 __global__ __launch_bounds__(64) void cache_train_paths_kernel(CacheFrame p) {
-    __shared__ uint32_t s_ref[kCK * 64];
-    __shared__ float s_dist[kCK * 64];
+    __shared__ uint2 s_stack[kCK * 64];
     const uint32_t lane = threadIdx.x;
     SpillArray spill;
     Stack<kCK> st;
-    st.spill = spill;
-    st.ref = (lds_u32 *)(s_ref + lane);
-    st.dist = (lds_f32 *)(s_dist + lane);
+    st.init(s_stack, lane, spill);
     LaneCounters c{0, 0, 0, 0, 0};
     const uint32_t npix = p.width * p.height;
     const uint32_t stride = npix / p.train_paths > 0 ? npix / p.train_paths : 1u;

@@ -24,14 +24,17 @@ nb_status sobol_generate(uint32_t rows, uint32_t dims, uint32_t seed, float *out
 
 // ---- device layout (HBM) --------------------------------------------------------------------
 //
-// Node pairs: one 64-B record per inner BVH node holding both children's boxes, so a traversal
-// step is one record fetch.  Root records hold the root box as the left child and an all-NaN
-// right box (the slab test misses NaN boxes).  BLAS and TLAS records share one array.
+// Node records: one 64-B child pair per inner BVH node holding both children's boxes and
+// references, so a traversal step is one record fetch (four dwordx4 loads).  Root records hold
+// the root box as the left child and an all-NaN right box (the slab test misses NaN boxes), so
+// entering a tree is the same step as any other.  BLAS and TLAS records share one array, each
+// tree in depth-first preorder after its root record.
 //   q0 = (L.min.xyz, L.max.x)  q1 = (L.max.yz, R.min.xy)  q2 = (R.min.z, R.max.xyz)
 //   q3 = (L.ref, R.ref, is_root, 0)  as bits
 // A child reference is a record id (bit 31 clear) or a leaf: bit 31 set, primitive count in
 // bits 26..30 (0 = more than 31: the count sits in the leaf-count side table at the leaf's
 // start), first primitive in bits 0..25.
+constexpr uint32_t kRecordFloat4 = 4;
 constexpr uint32_t kLeafBit = 0x80000000u;
 constexpr uint32_t kLeafStartBits = 26;
 constexpr uint32_t kLeafStartMask = (1u << kLeafStartBits) - 1u;
@@ -85,9 +88,10 @@ struct RenderParams {
     unsigned long long *counters;  // rays, n_aabb, n_tri, n_inst, samples
     uint32_t tiles_x, n_tiles, tiles_per_xcd;
     uint32_t *hits_out;      // primary-hit debug output (5 words per pixel) or null
-    uint32_t *work;          // 8 per-XCD tile counters of the persistent kernel, zeroed per launch
+    uint32_t *work;          // 8 per-XCD work-unit counters of the persistent kernel, zeroed per launch
     uint32_t regen_min_idle; // the persistent kernel refills idle lanes once at least this many are idle
     uint32_t inner_quorum;   // inner-node steps run alone while more lanes than this have one
+    uint32_t tail_units;     // single work units are handed out once a band has at most this many
 };
 
 // ---- serving (serve.hip, server.cpp) ----------------------------------------------------------

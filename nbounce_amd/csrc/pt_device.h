@@ -117,46 +117,48 @@ __device__ __forceinline__ void flush_counters(unsigned long long *counters, con
 
 struct Hit { float t, u, v; uint32_t tri, inst; };
 
-// LDS (address space 3) views, so stack accesses compile to ds_read / ds_write.
-typedef __attribute__((address_space(3))) uint32_t lds_u32;
-typedef __attribute__((address_space(3))) float lds_f32;
+// LDS (address space 3) view, so stack accesses compile to ds_read_b64 / ds_write_b64.
+typedef __attribute__((address_space(3))) uint2 lds_uint2;
 
 // Traversal stack: top K entries per lane in LDS, older entries in a private array owned by the
 // kernel (declared beside the Stack, so the Stack's own fields stay in registers: a struct that
-// holds a dynamically indexed array is placed in scratch memory as a whole).
+// holds a dynamically indexed array is placed in scratch memory as a whole).  An entry is
+// (reference, distance bits); a lane's slot s sits at s * 64 + lane of the wave's LDS array, so
+// the 64 lanes of a push or pop touch consecutive 8-byte words.
 typedef uint2 SpillArray[kStackEntries];
 
 template <int K>
 struct Stack {
-    lds_u32 *ref;   // slot s of this lane at ref[s * 64]
-    lds_f32 *dist;
-    uint2 *spill;   // kStackEntries entries
+    lds_uint2 *ent;   // slot s of this lane at ent[s * 64]
+    uint2 *spill;     // kStackEntries entries
     uint32_t sp, lo;
 
+    __device__ __forceinline__ void init(uint2 *lds_base, uint32_t lane, uint2 *spill_array) {
+        ent = (lds_uint2 *)(lds_base + lane);
+        spill = spill_array;
+        sp = 0;
+        lo = 0;
+    }
     __device__ __forceinline__ void push(uint32_t r, float t) {
         if (sp >= kStackEntries) return;  // unreachable for trees of depth <= 32 (checked at upload)
         if (sp - lo == static_cast<uint32_t>(K)) {
-            uint32_t s = lo & (K - 1);
-            spill[lo] = make_uint2(ref[s * 64], __float_as_uint(dist[s * 64]));
+            spill[lo] = ent[(lo & (K - 1)) * 64];
             ++lo;
         }
-        uint32_t s = sp & (K - 1);
-        ref[s * 64] = r;
-        dist[s * 64] = t;
+        ent[(sp & (K - 1)) * 64] = make_uint2(r, __float_as_uint(t));
         ++sp;
     }
     __device__ __forceinline__ void pop(uint32_t &r, float &t) {
         --sp;
+        uint2 e;
         if (sp < lo) {
-            uint2 e = spill[sp];
+            e = spill[sp];
             lo = sp;
-            r = e.x;
-            t = __uint_as_float(e.y);
         } else {
-            uint32_t s = sp & (K - 1);
-            r = ref[s * 64];
-            t = dist[s * 64];
+            e = ent[(sp & (K - 1)) * 64];
         }
+        r = e.x;
+        t = __uint_as_float(e.y);
     }
 };
 
@@ -212,13 +214,11 @@ __device__ __forceinline__ void enter_instance(const DevScene &S, const Ray &wor
     ts.have = true;
 }
 
-// The inner-node part of a traversal step at record ts.cur: test both children, push the far one
-// if it is nearer than the current hit, continue with the near one (have = true) or not.
+// The descent at a child pair q0..q3: test both children, push the far one if it is nearer than
+// the current hit, continue with the near one (have = true) or not.
 template <int K>
-__device__ __forceinline__ void visit_inner(const DevScene &S, TraceState &ts, Stack<K> &st, LaneCounters &c) {
-    const float4 *rec = S.pairs + 4 * static_cast<size_t>(ts.cur);
-    float4 q0 = rec[0], q1 = rec[1], q2 = rec[2];
-    uint4 q3 = *reinterpret_cast<const uint4 *>(rec + 3);
+__device__ __forceinline__ void descend(TraceState &ts, Stack<K> &st, LaneCounters &c, const float4 &q0,
+                                       const float4 &q1, const float4 &q2, const uint4 &q3) {
     float tl = slab(ts.ray, q0.x, q0.y, q0.z, q0.w, q1.x, q1.y);
     float tr = slab(ts.ray, q1.z, q1.w, q2.x, q2.y, q2.z, q2.w);
     c.aabb += q3.z ? 1u : 2u;
@@ -229,6 +229,15 @@ __device__ __forceinline__ void visit_inner(const DevScene &S, TraceState &ts, S
     if (ft < ts.hit.t) st.push(fref, ft);
     ts.have = nt < ts.hit.t;
     if (ts.have) ts.cur = nref;
+}
+
+// The inner-node part of a traversal step at record ts.cur.
+template <int K>
+__device__ __forceinline__ void visit_inner(const DevScene &S, TraceState &ts, Stack<K> &st, LaneCounters &c) {
+    const float4 *rec = S.pairs + kRecordFloat4 * static_cast<size_t>(ts.cur);
+    const float4 q0 = rec[0], q1 = rec[1], q2 = rec[2];
+    const uint4 q3 = *reinterpret_cast<const uint4 *>(rec + 3);
+    descend<K>(ts, st, c, q0, q1, q2, q3);
 }
 
 // True when the next action of a lane is an inner-node visit or a pop that neither leaves a

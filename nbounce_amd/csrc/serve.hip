@@ -64,8 +64,7 @@ __global__ __launch_bounds__(256) void serve_generate_kernel(const ServeStep *st
 }
 
 __global__ __launch_bounds__(64) void serve_bounce_kernel(const ServeStep *step, uint32_t level) {
-    __shared__ uint32_t s_ref[kServeK * 64];
-    __shared__ float s_dist[kServeK * 64];
+    __shared__ uint2 s_stack[kServeK * 64];
     const ServeStep &S = *step;
     const uint32_t lane = threadIdx.x;
     const uint32_t n_in = S.level[2 * level];
@@ -73,9 +72,7 @@ __global__ __launch_bounds__(64) void serve_bounce_kernel(const ServeStep *step,
     float4 *out = S.queue[(level + 1u) & 1u];
     SpillArray spill;
     Stack<kServeK> st;
-    st.spill = spill;
-    st.ref = (lds_u32 *)(s_ref + lane);
-    st.dist = (lds_f32 *)(s_dist + lane);
+    st.init(s_stack, lane, spill);
     LaneCounters c{0, 0, 0, 0, 0};
     uint32_t samples_done = 0;
     const SobolView sob{S.sobol, S.sobol_entries, S.sobol_stride};

@@ -33,17 +33,14 @@ __device__ __forceinline__ uint32_t tile_of_block(const RenderParams &p) {
 // main, pathtracing.wgsl:243-260, for frames first_sample .. first_sample + n_frames - 1.
 template <int K>
 __global__ __launch_bounds__(64) void render_kernel(RenderParams p) {
-    __shared__ uint32_t s_ref[K * 64];
-    __shared__ float s_dist[K * 64];
+    __shared__ uint2 s_stack[K * 64];
     const uint32_t tile = tile_of_block(p);
     if (tile >= p.n_tiles) return;
     const uint32_t lane = threadIdx.x;
     const uint32_t x = (tile % p.tiles_x) * 8u + (lane & 7u), y = (tile / p.tiles_x) * 8u + (lane >> 3);
     SpillArray spill;
     Stack<K> st;
-    st.spill = spill;
-    st.ref = (lds_u32 *)(s_ref + lane);
-    st.dist = (lds_f32 *)(s_dist + lane);
+    st.init(s_stack, lane, spill);
     LaneCounters c{0, 0, 0, 0, 0};
     float4 *px = p.target + static_cast<size_t>(y) * p.width + x;
     PathConst pc;
@@ -69,22 +66,34 @@ __global__ __launch_bounds__(64) void render_kernel(RenderParams p) {
     flush_counters(p.counters, c, p.n_frames);
 }
 
-// Next tile for a wave on XCD `xcd`: its own band of tiles first, then the other XCDs' bands
-// (work stealing at the tail).  Returns n_tiles when every band is exhausted.  Wave-uniform.
-__device__ __forceinline__ uint32_t grab_tile(const RenderParams &p, uint32_t xcd) {
-    uint32_t t = p.n_tiles;
+// Work units of the persistent kernel: 16 pixels (two rows of an 8x8 tile), counted per XCD
+// band of tiles in p.work.  A wave takes the next units of its own XCD's band, then of the other
+// bands (work stealing at the tail): 4 units (a whole tile) at a time while more than
+// p.tail_units units remain in the band, one unit at a time after that, so the pixels still
+// queued in waves when the frame runs out are few.  Sets the wave-uniform pixel range
+// [begin, end) in tile order (pixel i is pixel i % 64 of tile i / 64); empty when all bands are
+// exhausted.
+__device__ __forceinline__ void grab_units(const RenderParams &p, uint32_t xcd, uint32_t &begin, uint32_t &end) {
+    uint32_t b0 = 0, b1 = 0;
     if (threadIdx.x == 0) {
         for (uint32_t k = 0; k < 8u; ++k) {
             const uint32_t b = (xcd + k) & 7u;
             const uint32_t lo = b * p.tiles_per_xcd;
             if (lo >= p.n_tiles) continue;
-            const uint32_t size = min(p.tiles_per_xcd, p.n_tiles - lo);
-            if (__hip_atomic_load(p.work + b, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= size) continue;
-            const uint32_t i = atomicAdd(p.work + b, 1u);
-            if (i < size) { t = lo + i; break; }
+            const uint32_t units = min(p.tiles_per_xcd, p.n_tiles - lo) * 4u;
+            const uint32_t seen = __hip_atomic_load(p.work + b, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (seen >= units) continue;
+            const uint32_t want = units - seen > p.tail_units ? 4u : 1u;
+            const uint32_t i = atomicAdd(p.work + b, want);
+            if (i < units) {
+                b0 = (lo * 4u + i) * 16u;
+                b1 = (lo * 4u + min(i + want, units)) * 16u;
+                break;
+            }
         }
     }
-    return __shfl(t, 0, 64);
+    begin = __shfl(b0, 0, 64);
+    end = __shfl(b1, 0, 64);
 }
 
 // Persistent form of render_kernel with ray-level regeneration.  A fixed grid of wave64 workers
@@ -101,16 +110,13 @@ __device__ __forceinline__ uint32_t grab_tile(const RenderParams &p, uint32_t xc
 // render_kernel, and the image does not depend on which lane or wave renders a pixel.
 template <int K, bool kCount>
 __global__ __launch_bounds__(64) void render_pt_kernel(RenderParams p) {
-    __shared__ uint32_t s_ref[K * 64];
-    __shared__ float s_dist[K * 64];
+    __shared__ uint2 s_stack[K * 64];
     const uint32_t lane = threadIdx.x;
     const uint32_t xcd = blockIdx.x & 7u;
     const uint32_t min_service = p.regen_min_idle;
     SpillArray spill;
     Stack<K> st;
-    st.spill = spill;
-    st.ref = (lds_u32 *)(s_ref + lane);
-    st.dist = (lds_f32 *)(s_dist + lane);
+    st.init(s_stack, lane, spill);
     LaneCounters c{0, 0, 0, 0, 0};
     PathConst pc;
     pc.scene = p.scene;
@@ -119,9 +125,9 @@ __global__ __launch_bounds__(64) void render_pt_kernel(RenderParams p) {
     pc.bounces = p.bounces;
     pc.contribution_factor = p.contribution_factor;
 
-    // wave-uniform tile stream: pixels [pos, 64) of tile `cur` come next, then tile `nxt`
-    uint32_t cur = grab_tile(p, xcd), pos = 0;
-    uint32_t nxt = cur < p.n_tiles ? grab_tile(p, xcd) : p.n_tiles;
+    // wave-uniform pixel stream: pixels [pos, end) of the wave's current units come next
+    uint32_t pos = 0, end = 0;
+    bool more = true;   // the bands may still hold units
 
     enum : uint32_t { kIdle = 0, kTracing = 1, kReady = 2 };
     uint32_t mode = kIdle;
@@ -169,15 +175,21 @@ __global__ __launch_bounds__(64) void render_pt_kernel(RenderParams p) {
         }
         // ---- service: idle lanes take the next pixels of the stream, in lane order
         const unsigned long long idle = __ballot(mode == kIdle);
-        if (idle != 0ull && cur < p.n_tiles) {
+        if (idle != 0ull && more) {
             const uint32_t n_idle = static_cast<uint32_t>(__popcll(idle));
-            const uint32_t avail = 64u - pos;
+            const uint32_t avail = end - pos;
+            uint32_t nb = 0, ne = 0;   // units taken now, for the lanes the current ones do not cover
+            if (avail < n_idle) {
+                grab_units(p, xcd, nb, ne);
+                if (nb == ne) more = false;
+            }
             if (mode == kIdle) {
                 const uint32_t rank = static_cast<uint32_t>(__popcll(idle & ((1ull << lane) - 1ull)));
-                uint32_t tile = p.n_tiles, k = 0;
-                if (rank < avail) { tile = cur; k = pos + rank; }
-                else if (nxt < p.n_tiles) { tile = nxt; k = rank - avail; }
-                if (tile < p.n_tiles) {
+                uint32_t pix = 0xFFFFFFFFu;
+                if (rank < avail) pix = pos + rank;
+                else if (rank - avail < ne - nb) pix = nb + (rank - avail);
+                if (pix != 0xFFFFFFFFu) {
+                    const uint32_t tile = pix >> 6, k = pix & 63u;
                     x = (tile % p.tiles_x) * 8u + (k & 7u);
                     y = (tile / p.tiles_x) * 8u + (k >> 3);
                     mode = kTracing;
@@ -193,12 +205,13 @@ __global__ __launch_bounds__(64) void render_pt_kernel(RenderParams p) {
                     }
                 }
             }
-            if (n_idle < avail) {
+            if (n_idle <= avail) {
                 pos += n_idle;
+            } else if (ne > nb) {
+                pos = nb + min(n_idle - avail, ne - nb);
+                end = ne;
             } else {
-                cur = nxt;
-                pos = n_idle - avail;   // < 64: pixels of the new current tile already taken
-                nxt = cur < p.n_tiles ? grab_tile(p, xcd) : p.n_tiles;
+                pos = end;
             }
         }
         // ---- start the traversal of new rays (camera rays at bounce 0)
@@ -214,7 +227,7 @@ __global__ __launch_bounds__(64) void render_pt_kernel(RenderParams p) {
         // every lane idle after the refill: the stream is exhausted
         if (__ballot(mode != kIdle) == 0ull) break;
         // ---- traverse
-        const bool more_pixels = cur < p.n_tiles;
+        const bool more_pixels = more;
         for (;;) {
             // inner-node visits and plain pops of every lane that has one, until no more than
             // inner_quorum lanes do: lanes that reached a leaf, a BLAS exit or the end wait, so
@@ -240,17 +253,14 @@ __global__ void reset_work_kernel(uint32_t *work) {
 
 template <int K>
 __global__ __launch_bounds__(64) void primary_hits_kernel(RenderParams p) {
-    __shared__ uint32_t s_ref[K * 64];
-    __shared__ float s_dist[K * 64];
+    __shared__ uint2 s_stack[K * 64];
     const uint32_t tile = tile_of_block(p);
     if (tile >= p.n_tiles) return;
     const uint32_t lane = threadIdx.x;
     const uint32_t x = (tile % p.tiles_x) * 8u + (lane & 7u), y = (tile / p.tiles_x) * 8u + (lane >> 3);
     SpillArray spill;
     Stack<K> st;
-    st.spill = spill;
-    st.ref = (lds_u32 *)(s_ref + lane);
-    st.dist = (lds_f32 *)(s_dist + lane);
+    st.init(s_stack, lane, spill);
     LaneCounters c{0, 0, 0, 0, 0};
     const float4 shift = hash4f(x, y, x, y);
     float4 jit = sobol_fetch(SobolView{p.sobol, p.sobol_entries, p.sobol_stride}, p.first_sample * p.sobol_stride, shift);
@@ -329,6 +339,15 @@ static uint32_t inner_quorum() {
     return v;
 }
 
+// NB_TAIL_UNITS=n: single 16-pixel units are handed out once a band has at most n left.
+static uint32_t tail_units() {
+    static const uint32_t v = [] {
+        const char *e = std::getenv("NB_TAIL_UNITS");
+        return e ? static_cast<uint32_t>(std::atoi(e)) : 2048u;
+    }();
+    return v;
+}
+
 // NB_REGEN_MIN_IDLE=n (1..64): a wave leaves its traversal phase to shade and refill once at
 // least n lanes wait (finished hit, or idle while pixels remain).
 static uint32_t regen_min_idle() {
@@ -345,6 +364,7 @@ static nb_status launch_regen(const RenderParams &p_in, hipStream_t s) {
     RenderParams p = p_in;
     p.regen_min_idle = regen_min_idle();
     p.inner_quorum = inner_quorum();
+    p.tail_units = tail_units();
     const uint32_t grid = regen_grid();
     if (grid == 0) return set_error(NB_ERR_HIP, "render: device attribute query failed");
     hipLaunchKernelGGL(reset_work_kernel, dim3(1), dim3(64), 0, s, p.work);
