@@ -59,7 +59,7 @@ __device__ __forceinline__ uint32_t wave_prefix_alloc(bool want, uint32_t *count
 // max_cache_bounce; there it stops and queues a cache query with its throughput.  Radiance met
 // before (sky, emitters) goes to `direct`.
 __global__ __launch_bounds__(64) void cache_query_kernel(CacheFrame p) {
-    __shared__ uint2 s_stack[kCK * 64];
+    __shared__ unsigned long long s_stack[kCK * 64];
     const uint32_t tile = (blockIdx.x & 7u) * p.tiles_per_xcd + (blockIdx.x >> 3);
     if (tile >= p.n_tiles) return;
     const uint32_t lane = threadIdx.x;
@@ -111,7 +111,7 @@ __global__ __launch_bounds__(64) void cache_query_kernel(CacheFrame p) {
 // a training record whose target is the path's final radiance divided by the throughput that
 // arrived at that point, i.e. the one-sample estimate of the point's outgoing radiance.
 __global__ __launch_bounds__(64) void cache_train_paths_kernel(CacheFrame p) {
-    __shared__ uint2 s_stack[kCK * 64];
+    __shared__ unsigned long long s_stack[kCK * 64];
     const uint32_t lane = threadIdx.x;
     SpillArray spill;
     Stack<kCK> st;

@@ -92,6 +92,7 @@ struct RenderParams {
     uint32_t regen_min_idle; // the persistent kernel refills idle lanes once at least this many are idle
     uint32_t inner_quorum;   // inner-node steps run alone while more lanes than this have one
     uint32_t tail_units;     // single work units are handed out once a band has at most this many
+    uint32_t grab_units;     // work units per grab before that
 };
 
 // ---- serving (serve.hip, server.cpp) ----------------------------------------------------------

@@ -118,23 +118,23 @@ __device__ __forceinline__ void flush_counters(unsigned long long *counters, con
 struct Hit { float t, u, v; uint32_t tri, inst; };
 
 // LDS (address space 3) view, so stack accesses compile to ds_read_b64 / ds_write_b64.
-typedef __attribute__((address_space(3))) uint2 lds_uint2;
+typedef __attribute__((address_space(3))) unsigned long long lds_u64;
 
 // Traversal stack: top K entries per lane in LDS, older entries in a private array owned by the
 // kernel (declared beside the Stack, so the Stack's own fields stay in registers: a struct that
-// holds a dynamically indexed array is placed in scratch memory as a whole).  An entry is
-// (reference, distance bits); a lane's slot s sits at s * 64 + lane of the wave's LDS array, so
-// the 64 lanes of a push or pop touch consecutive 8-byte words.
-typedef uint2 SpillArray[kStackEntries];
+// holds a dynamically indexed array is placed in scratch memory as a whole).  An entry is one
+// 64-bit word, distance bits high and reference low; a lane's slot s sits at s * 64 + lane of the
+// wave's LDS array, so the 64 lanes of a push or pop touch consecutive words.
+typedef unsigned long long SpillArray[kStackEntries];
 
 template <int K>
 struct Stack {
-    lds_uint2 *ent;   // slot s of this lane at ent[s * 64]
-    uint2 *spill;     // kStackEntries entries
+    lds_u64 *ent;                 // slot s of this lane at ent[s * 64]
+    unsigned long long *spill;    // kStackEntries entries
     uint32_t sp, lo;
 
-    __device__ __forceinline__ void init(uint2 *lds_base, uint32_t lane, uint2 *spill_array) {
-        ent = (lds_uint2 *)(lds_base + lane);
+    __device__ __forceinline__ void init(unsigned long long *lds_base, uint32_t lane, unsigned long long *spill_array) {
+        ent = (lds_u64 *)(lds_base + lane);
         spill = spill_array;
         sp = 0;
         lo = 0;
@@ -145,20 +145,20 @@ struct Stack {
             spill[lo] = ent[(lo & (K - 1)) * 64];
             ++lo;
         }
-        ent[(sp & (K - 1)) * 64] = make_uint2(r, __float_as_uint(t));
+        ent[(sp & (K - 1)) * 64] = (static_cast<unsigned long long>(__float_as_uint(t)) << 32) | r;
         ++sp;
     }
     __device__ __forceinline__ void pop(uint32_t &r, float &t) {
         --sp;
-        uint2 e;
+        unsigned long long e;
         if (sp < lo) {
             e = spill[sp];
             lo = sp;
         } else {
             e = ent[(sp & (K - 1)) * 64];
         }
-        r = e.x;
-        t = __uint_as_float(e.y);
+        r = static_cast<uint32_t>(e);
+        t = __uint_as_float(static_cast<uint32_t>(e >> 32));
     }
 };
 

@@ -64,7 +64,7 @@ __global__ __launch_bounds__(256) void serve_generate_kernel(const ServeStep *st
 }
 
 __global__ __launch_bounds__(64) void serve_bounce_kernel(const ServeStep *step, uint32_t level) {
-    __shared__ uint2 s_stack[kServeK * 64];
+    __shared__ unsigned long long s_stack[kServeK * 64];
     const ServeStep &S = *step;
     const uint32_t lane = threadIdx.x;
     const uint32_t n_in = S.level[2 * level];

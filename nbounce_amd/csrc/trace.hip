@@ -33,7 +33,7 @@ __device__ __forceinline__ uint32_t tile_of_block(const RenderParams &p) {
 // main, pathtracing.wgsl:243-260, for frames first_sample .. first_sample + n_frames - 1.
 template <int K>
 __global__ __launch_bounds__(64) void render_kernel(RenderParams p) {
-    __shared__ uint2 s_stack[K * 64];
+    __shared__ unsigned long long s_stack[K * 64];
     const uint32_t tile = tile_of_block(p);
     if (tile >= p.n_tiles) return;
     const uint32_t lane = threadIdx.x;
@@ -68,9 +68,9 @@ __global__ __launch_bounds__(64) void render_kernel(RenderParams p) {
 
 // Work units of the persistent kernel: 16 pixels (two rows of an 8x8 tile), counted per XCD
 // band of tiles in p.work.  A wave takes the next units of its own XCD's band, then of the other
-// bands (work stealing at the tail): 4 units (a whole tile) at a time while more than
-// p.tail_units units remain in the band, one unit at a time after that, so the pixels still
-// queued in waves when the frame runs out are few.  Sets the wave-uniform pixel range
+// bands (work stealing at the tail): p.grab_units units at a time while more than p.tail_units
+// units remain in the band, one unit at a time after that.  Device-scope atomics are costly
+// (about 30 ns each on one address), so grabs are coarse.  Sets the wave-uniform pixel range
 // [begin, end) in tile order (pixel i is pixel i % 64 of tile i / 64); empty when all bands are
 // exhausted.
 __device__ __forceinline__ void grab_units(const RenderParams &p, uint32_t xcd, uint32_t &begin, uint32_t &end) {
@@ -83,7 +83,7 @@ __device__ __forceinline__ void grab_units(const RenderParams &p, uint32_t xcd, 
             const uint32_t units = min(p.tiles_per_xcd, p.n_tiles - lo) * 4u;
             const uint32_t seen = __hip_atomic_load(p.work + b, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             if (seen >= units) continue;
-            const uint32_t want = units - seen > p.tail_units ? 4u : 1u;
+            const uint32_t want = units - seen > p.tail_units ? p.grab_units : 1u;
             const uint32_t i = atomicAdd(p.work + b, want);
             if (i < units) {
                 b0 = (lo * 4u + i) * 16u;
@@ -110,7 +110,7 @@ __device__ __forceinline__ void grab_units(const RenderParams &p, uint32_t xcd, 
 // render_kernel, and the image does not depend on which lane or wave renders a pixel.
 template <int K, bool kCount>
 __global__ __launch_bounds__(64) void render_pt_kernel(RenderParams p) {
-    __shared__ uint2 s_stack[K * 64];
+    __shared__ unsigned long long s_stack[K * 64];
     const uint32_t lane = threadIdx.x;
     const uint32_t xcd = blockIdx.x & 7u;
     const uint32_t min_service = p.regen_min_idle;
@@ -253,7 +253,7 @@ __global__ void reset_work_kernel(uint32_t *work) {
 
 template <int K>
 __global__ __launch_bounds__(64) void primary_hits_kernel(RenderParams p) {
-    __shared__ uint2 s_stack[K * 64];
+    __shared__ unsigned long long s_stack[K * 64];
     const uint32_t tile = tile_of_block(p);
     if (tile >= p.n_tiles) return;
     const uint32_t lane = threadIdx.x;
@@ -343,7 +343,17 @@ static uint32_t inner_quorum() {
 static uint32_t tail_units() {
     static const uint32_t v = [] {
         const char *e = std::getenv("NB_TAIL_UNITS");
-        return e ? static_cast<uint32_t>(std::atoi(e)) : 2048u;
+        return e ? static_cast<uint32_t>(std::atoi(e)) : 0u;
+    }();
+    return v;
+}
+
+// NB_GRAB_UNITS=n: 16-pixel units per grab before the tail.
+static uint32_t grab_units_setting() {
+    static const uint32_t v = [] {
+        const char *e = std::getenv("NB_GRAB_UNITS");
+        const int n = e ? std::atoi(e) : 4;
+        return static_cast<uint32_t>(n < 1 ? 1 : (n > 256 ? 256 : n));
     }();
     return v;
 }
@@ -365,6 +375,7 @@ static nb_status launch_regen(const RenderParams &p_in, hipStream_t s) {
     p.regen_min_idle = regen_min_idle();
     p.inner_quorum = inner_quorum();
     p.tail_units = tail_units();
+    p.grab_units = grab_units_setting();
     const uint32_t grid = regen_grid();
     if (grid == 0) return set_error(NB_ERR_HIP, "render: device attribute query failed");
     hipLaunchKernelGGL(reset_work_kernel, dim3(1), dim3(64), 0, s, p.work);
