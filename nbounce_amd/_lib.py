@@ -10,7 +10,8 @@ import numpy as np
 
 from . import layout
 
-LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libnbounce.so")
+# NB_LIBRARY points at another build of the same library (for comparing builds); default: in-tree
+LIB_PATH = os.environ.get("NB_LIBRARY") or os.path.join(os.path.dirname(os.path.abspath(__file__)), "libnbounce.so")
 
 _c = ctypes
 _p = _c.c_void_p

@@ -162,6 +162,15 @@ nb_status relayout(const nb_bvh_node *blas, uint32_t n_blas, const nb_bvh_node *
 
     R.tri_leaf.assign(n_tris ? n_tris : 1, 0u);
     R.inst_leaf.assign(n_inst ? n_inst : 1, 0u);
+    auto pair_of = [&](const nb_bvh_node *nodes, std::vector<uint32_t> &pid, std::vector<uint32_t> &side, uint32_t i,
+                       float4 *q) -> nb_status {
+        const uint32_t l = nodes[i].start, r = l + 1;
+        uint32_t lr, rr;
+        NB_TRY(child_ref(nodes[l], pid, l, side, &lr));
+        NB_TRY(child_ref(nodes[r], pid, r, side, &rr));
+        make_pair(nodes[l], lr, &nodes[r], rr, 0u, q);
+        return NB_OK;
+    };
     R.pairs.assign(kRecordFloat4 * recs.size(), make_float4(0.0f, 0.0f, 0.0f, 0.0f));
     for (size_t k = 0; k < recs.size(); ++k) {
         const Rec &rc = recs[k];
@@ -175,11 +184,7 @@ nb_status relayout(const nb_bvh_node *blas, uint32_t n_blas, const nb_bvh_node *
             NB_TRY(child_ref(nodes[rc.node], pid, rc.node, side, &ref));
             make_pair(nodes[rc.node], ref, nullptr, 0u, 1u, q);
         } else {
-            const uint32_t l = nodes[rc.node].start, r = l + 1;
-            uint32_t lr, rr;
-            NB_TRY(child_ref(nodes[l], pid, l, side, &lr));
-            NB_TRY(child_ref(nodes[r], pid, r, side, &rr));
-            make_pair(nodes[l], lr, &nodes[r], rr, 0u, q);
+            NB_TRY(pair_of(nodes, pid, side, rc.node, q));
         }
     }
 
@@ -357,8 +362,8 @@ nb_status nb_context_create(int device, nb_context **out) {
     if (e == hipSuccess) e = hipEventCreate(&c->ev_region_stop);
     if (e == hipSuccess) e = hipMalloc(&c->counters, kCounterBytes);
     if (e == hipSuccess) e = hipMemset(c->counters, 0, kCounterBytes);
-    if (e == hipSuccess) e = hipMalloc(&c->work, 16 * sizeof(uint32_t));
-    if (e == hipSuccess) e = hipMemset(c->work, 0, 16 * sizeof(uint32_t));
+    if (e == hipSuccess) e = hipMalloc(&c->work, kWorkBytes);
+    if (e == hipSuccess) e = hipMemset(c->work, 0, kWorkBytes);
     if (e != hipSuccess) {
         c->closing = true;
         release_context(c);
@@ -667,9 +672,11 @@ nb_status nb_counters_get(nb_context *ctx, nb_counters *out) {
     if (!ctx || !out) return set_error(NB_ERR_INVALID_ARGUMENT, "null argument");
     NB_TRY(use_device(ctx));
     NB_HIP(hipStreamSynchronize(ctx->stream));
-    unsigned long long v[kCounterSlots * kCounterWords], sum[kCounterWords] = {};
-    NB_HIP(hipMemcpy(v, ctx->counters, sizeof(v), hipMemcpyDeviceToHost));
-    for (uint32_t i = 0; i < kCounterSlots * kCounterWords; ++i) sum[i % kCounterWords] += v[i];
+    std::vector<unsigned long long> v(kCounterBytes / sizeof(unsigned long long));
+    unsigned long long sum[kCounterWords] = {};
+    NB_HIP(hipMemcpy(v.data(), ctx->counters, kCounterBytes, hipMemcpyDeviceToHost));
+    for (uint32_t k = 0; k < kCounterSlots; ++k)
+        for (uint32_t i = 0; i < kCounterWords; ++i) sum[i] += v[k * kCounterStride + i];
     out->rays = sum[0]; out->n_aabb = sum[1]; out->n_tri = sum[2]; out->n_inst = sum[3]; out->samples = sum[4];
     out->hits = sum[5];
     return NB_OK;

@@ -28,7 +28,8 @@ nb_status sobol_generate(uint32_t rows, uint32_t dims, uint32_t seed, float *out
 // references, so a traversal step is one record fetch (four dwordx4 loads).  Root records hold
 // the root box as the left child and an all-NaN right box (the slab test misses NaN boxes), so
 // entering a tree is the same step as any other.  BLAS and TLAS records share one array, each
-// tree in depth-first preorder after its root record.
+// tree in depth-first preorder after its root record.  (Records that also carried both
+// children's pairs, descending two levels per fetch, measured slower.)
 //   q0 = (L.min.xyz, L.max.x)  q1 = (L.max.yz, R.min.xy)  q2 = (R.min.z, R.max.xyz)
 //   q3 = (L.ref, R.ref, is_root, 0)  as bits
 // A child reference is a record id (bit 31 clear) or a leaf: bit 31 set, primitive count in
@@ -43,10 +44,17 @@ constexpr uint32_t kNoRecord = 0xFFFFFFFFu;
 constexpr uint32_t kMaxTreeDepth = 32;  // reference MAX_DEPTH (bvh.rs:153)
 constexpr uint32_t kStackEntries = 2 * kMaxTreeDepth;
 
-// Device counters: kCounterSlots slots (one per XCD) of kCounterWords u64 each:
+// Device counters: kCounterSlots slots (one per XCD), kCounterStride u64 apart (4 KiB, so the
+// slots' atomics do not share a cache line or memory channel), each of kCounterWords u64:
 // rays, n_aabb, n_tri, n_inst, samples, hits, 0, 0.  Readers sum the slots.
-constexpr uint32_t kCounterSlots = 8, kCounterWords = 8;
-constexpr size_t kCounterBytes = kCounterSlots * kCounterWords * sizeof(unsigned long long);
+constexpr uint32_t kCounterSlots = 8, kCounterWords = 8, kCounterStride = 512;
+constexpr size_t kCounterBytes = kCounterSlots * kCounterStride * sizeof(unsigned long long);
+
+// Per-XCD work counters of the persistent render kernel, kWorkStride words apart: device-scope
+// atomics on one cache line (or one memory channel) serialise, so each counter gets its own
+// 4 KiB block.
+constexpr uint32_t kWorkStride = 1024;
+constexpr size_t kWorkBytes = 8 * kWorkStride * sizeof(uint32_t);
 
 // Triangles in BVH (permuted) order, 48 B: (v0.xyz, i0) (e1.xyz, i1) (e2.xyz, i2) with
 // e1 = v1 - v0, e2 = v2 - v0 computed in binary32 exactly as the kernel would.
@@ -88,7 +96,7 @@ struct RenderParams {
     unsigned long long *counters;  // rays, n_aabb, n_tri, n_inst, samples
     uint32_t tiles_x, n_tiles, tiles_per_xcd;
     uint32_t *hits_out;      // primary-hit debug output (5 words per pixel) or null
-    uint32_t *work;          // 8 per-XCD work-unit counters of the persistent kernel, zeroed per launch
+    uint32_t *work;          // 8 per-XCD work-unit counters (kWorkStride apart), zeroed per launch
     uint32_t regen_min_idle; // the persistent kernel refills idle lanes once at least this many are idle
     uint32_t inner_quorum;   // inner-node steps run alone while more lanes than this have one
     uint32_t tail_units;     // single work units are handed out once a band has at most this many

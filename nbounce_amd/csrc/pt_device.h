@@ -105,7 +105,7 @@ __device__ __forceinline__ void flush_counters(unsigned long long *counters, con
     auto sum = [](uint32_t v) { for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64); return v; };
     const uint32_t r = sum(c.rays), a = sum(c.aabb), t = sum(c.tri), i = sum(c.inst), s = sum(samples), h = sum(c.hits);
     if ((threadIdx.x & 63u) == 0 && counters && (r | a | t | i | s | h)) {
-        unsigned long long *slot = counters + kCounterWords * (blockIdx.x % kCounterSlots);
+        unsigned long long *slot = counters + kCounterStride * (blockIdx.x % kCounterSlots);
         if (r) atomicAdd(slot + 0, static_cast<unsigned long long>(r));
         if (a) atomicAdd(slot + 1, static_cast<unsigned long long>(a));
         if (t) atomicAdd(slot + 2, static_cast<unsigned long long>(t));
@@ -214,11 +214,13 @@ __device__ __forceinline__ void enter_instance(const DevScene &S, const Ray &wor
     ts.have = true;
 }
 
-// The descent at a child pair q0..q3: test both children, push the far one if it is nearer than
-// the current hit, continue with the near one (have = true) or not.
+// The inner-node part of a traversal step at record ts.cur: test both children, push the far one
+// if it is nearer than the current hit, continue with the near one (have = true) or not.
 template <int K>
-__device__ __forceinline__ void descend(TraceState &ts, Stack<K> &st, LaneCounters &c, const float4 &q0,
-                                       const float4 &q1, const float4 &q2, const uint4 &q3) {
+__device__ __forceinline__ void visit_inner(const DevScene &S, TraceState &ts, Stack<K> &st, LaneCounters &c) {
+    const float4 *rec = S.pairs + kRecordFloat4 * static_cast<size_t>(ts.cur);
+    const float4 q0 = rec[0], q1 = rec[1], q2 = rec[2];
+    const uint4 q3 = *reinterpret_cast<const uint4 *>(rec + 3);
     float tl = slab(ts.ray, q0.x, q0.y, q0.z, q0.w, q1.x, q1.y);
     float tr = slab(ts.ray, q1.z, q1.w, q2.x, q2.y, q2.z, q2.w);
     c.aabb += q3.z ? 1u : 2u;
@@ -229,15 +231,6 @@ __device__ __forceinline__ void descend(TraceState &ts, Stack<K> &st, LaneCounte
     if (ft < ts.hit.t) st.push(fref, ft);
     ts.have = nt < ts.hit.t;
     if (ts.have) ts.cur = nref;
-}
-
-// The inner-node part of a traversal step at record ts.cur.
-template <int K>
-__device__ __forceinline__ void visit_inner(const DevScene &S, TraceState &ts, Stack<K> &st, LaneCounters &c) {
-    const float4 *rec = S.pairs + kRecordFloat4 * static_cast<size_t>(ts.cur);
-    const float4 q0 = rec[0], q1 = rec[1], q2 = rec[2];
-    const uint4 q3 = *reinterpret_cast<const uint4 *>(rec + 3);
-    descend<K>(ts, st, c, q0, q1, q2, q3);
 }
 
 // True when the next action of a lane is an inner-node visit or a pop that neither leaves a

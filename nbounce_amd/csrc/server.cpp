@@ -370,9 +370,11 @@ nb_status nb_server_stats_get(nb_server *s, nb_server_stats *out) {
     if (!s || !out) return set_error(NB_ERR_INVALID_ARGUMENT, "null argument");
     SRV_HIP(hipSetDevice(s->ctx->device));
     SRV_HIP(hipStreamSynchronize(s->stream));
-    unsigned long long v[kCounterSlots * kCounterWords], sum[kCounterWords] = {};
-    SRV_HIP(hipMemcpy(v, s->counters, sizeof(v), hipMemcpyDeviceToHost));
-    for (uint32_t i = 0; i < kCounterSlots * kCounterWords; ++i) sum[i % kCounterWords] += v[i];
+    std::vector<unsigned long long> v(kCounterBytes / sizeof(unsigned long long));
+    unsigned long long sum[kCounterWords] = {};
+    SRV_HIP(hipMemcpy(v.data(), s->counters, kCounterBytes, hipMemcpyDeviceToHost));
+    for (uint32_t k = 0; k < kCounterSlots; ++k)
+        for (uint32_t i = 0; i < kCounterWords; ++i) sum[i] += v[k * kCounterStride + i];
     std::memset(out, 0, sizeof(*out));
     out->steps = s->steps;
     out->rays = sum[0]; out->n_aabb = sum[1]; out->n_tri = sum[2]; out->n_inst = sum[3]; out->samples = sum[4];

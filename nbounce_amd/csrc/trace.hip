@@ -81,10 +81,11 @@ __device__ __forceinline__ void grab_units(const RenderParams &p, uint32_t xcd, 
             const uint32_t lo = b * p.tiles_per_xcd;
             if (lo >= p.n_tiles) continue;
             const uint32_t units = min(p.tiles_per_xcd, p.n_tiles - lo) * 4u;
-            const uint32_t seen = __hip_atomic_load(p.work + b, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            uint32_t *counter = p.work + b * kWorkStride;
+            const uint32_t seen = __hip_atomic_load(counter, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             if (seen >= units) continue;
             const uint32_t want = units - seen > p.tail_units ? p.grab_units : 1u;
-            const uint32_t i = atomicAdd(p.work + b, want);
+            const uint32_t i = atomicAdd(counter, want);
             if (i < units) {
                 b0 = (lo * 4u + i) * 16u;
                 b1 = (lo * 4u + min(i + want, units)) * 16u;
@@ -248,7 +249,7 @@ __global__ __launch_bounds__(64) void render_pt_kernel(RenderParams p) {
 }
 
 __global__ void reset_work_kernel(uint32_t *work) {
-    if (threadIdx.x < 8) work[threadIdx.x] = 0u;
+    if (threadIdx.x < 8) work[threadIdx.x * kWorkStride] = 0u;
 }
 
 template <int K>
