@@ -334,7 +334,7 @@ def main():
     traffic = None
     try:
         with open(os.path.join(ROOT, "profiles", "traffic.json")) as f:
-            key = "render_kernel instanced 1920x1080 12 bounces shared BLAS"
+            key = "render_pt_kernel instanced 1920x1080 12 bounces shared BLAS"
             if args.workload == "large" and not args.reference_blas and F == 1:
                 traffic = json.load(f).get(key, {}).get("hbm_bytes_per_launch")
     except OSError:
@@ -376,8 +376,8 @@ def main():
             "frac": round(achieved / PEAK_HBM_GBS, 5),
             "traffic": traffic,
             "traffic_source": "rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE, scaled by a 256 MiB copy "
-                              "(profiles/render_kernel_instanced_1080p_traffic.md)" if traffic else None,
-            "kernel": "render_kernel",
+                              "(profiles/render_pt_kernel_instanced_1080p_traffic.md)" if traffic else None,
+            "kernel": "render_pt_kernel" if os.environ.get("NB_RENDER", "pt") != "tile" else "render_kernel",
             "kernel_ms": round(kernel_ms, 4),
             "algorithmic_bytes_per_launch": int(bytes_per_launch),
         },
