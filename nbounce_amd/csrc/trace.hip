@@ -388,13 +388,11 @@ static nb_status launch_regen(const RenderParams &p_in, hipStream_t s) {
 }
 
 // NB_RENDER=tile selects the one-tile-per-wave kernel (no regeneration) for comparisons; the
-// persistent kernel is the default.  Both render the same image.
+// persistent kernel is the default.  Both render the same image.  Read at every launch, so a
+// process can compare the two.
 static bool use_tile_kernel() {
-    static const bool tile = [] {
-        const char *e = std::getenv("NB_RENDER");
-        return e && std::strcmp(e, "tile") == 0;
-    }();
-    return tile;
+    const char *e = std::getenv("NB_RENDER");
+    return e && std::strcmp(e, "tile") == 0;
 }
 
 nb_status launch_render(const RenderParams &p, void *stream) {
