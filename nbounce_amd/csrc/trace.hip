@@ -126,9 +126,21 @@ __global__ __launch_bounds__(64) void render_pt_kernel(RenderParams p) {
     pc.bounces = p.bounces;
     pc.contribution_factor = p.contribution_factor;
 
-    // wave-uniform pixel stream: pixels [pos, end) of the wave's current units come next
+    // wave-uniform pixel stream: pixels [pos, end) of the wave's current units come next; the
+    // first units are this wave's static share of its XCD's band
     uint32_t pos = 0, end = 0;
     bool more = true;   // the bands may still hold units
+    {
+        const uint32_t lo = xcd * p.tiles_per_xcd;
+        if (lo < p.n_tiles) {
+            const uint32_t units = min(p.tiles_per_xcd, p.n_tiles - lo) * 4u;
+            const uint32_t u0 = (blockIdx.x >> 3) * p.grab_units;
+            if (u0 < units) {
+                pos = (lo * 4u + u0) * 16u;
+                end = (lo * 4u + min(u0 + p.grab_units, units)) * 16u;
+            }
+        }
+    }
 
     enum : uint32_t { kIdle = 0, kTracing = 1, kReady = 2 };
     uint32_t mode = kIdle;
@@ -248,8 +260,8 @@ __global__ __launch_bounds__(64) void render_pt_kernel(RenderParams p) {
     if (kCount) flush_counters(p.counters, c, frames_done);   // else the counts are dead code
 }
 
-__global__ void reset_work_kernel(uint32_t *work) {
-    if (threadIdx.x < 8) work[threadIdx.x * kWorkStride] = 0u;
+__global__ void reset_work_kernel(uint32_t *work, uint32_t first) {
+    if (threadIdx.x < 8) work[threadIdx.x * kWorkStride] = first;
 }
 
 template <int K>
@@ -312,17 +324,18 @@ template <int K> struct RenderKernel { static constexpr auto fn = render_kernel<
 template <int K> struct HitsKernel { static constexpr auto fn = primary_hits_kernel<K>; };
 }  // namespace
 
-// Workgroups of the persistent kernel: 32 one-wave workgroups per CU (the most a CU holds);
-// the hardware starts what fits and the rest start as residents finish, find the tile counters
-// exhausted and exit.  NB_REGEN_WAVES_PER_CU overrides the 32.  A multiple of 8 (XCDs).
+// Workgroups of the persistent kernel: as many one-wave workgroups as the CUs hold at once
+// (the occupancy the compiled kernel allows), a multiple of 8 (XCDs), so every wave's first,
+// statically assigned work units start at once.  NB_REGEN_WAVES_PER_CU overrides the per-CU count.
+template <int K, bool kCount>
 static uint32_t regen_grid() {
     static uint32_t grid = [] {
-        int dev = 0, cus = 0;
+        int dev = 0, cus = 0, per_cu = 0;
         if (hipGetDevice(&dev) != hipSuccess ||
-            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+            hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, render_pt_kernel<K, kCount>, 64, 0) != hipSuccess)
             return 0u;
-        const char *e = std::getenv("NB_REGEN_WAVES_PER_CU");
-        const int per_cu = e ? std::atoi(e) : 32;
+        if (const char *e = std::getenv("NB_REGEN_WAVES_PER_CU")) per_cu = std::atoi(e);
         const uint32_t g = static_cast<uint32_t>(cus) * static_cast<uint32_t>(per_cu < 1 ? 1 : per_cu) / 8u * 8u;
         return g < 8u ? 8u : g;
     }();
@@ -377,9 +390,12 @@ static nb_status launch_regen(const RenderParams &p_in, hipStream_t s) {
     p.inner_quorum = inner_quorum();
     p.tail_units = tail_units();
     p.grab_units = grab_units_setting();
-    const uint32_t grid = regen_grid();
-    if (grid == 0) return set_error(NB_ERR_HIP, "render: device attribute query failed");
-    hipLaunchKernelGGL(reset_work_kernel, dim3(1), dim3(64), 0, s, p.work);
+    const uint32_t grid = p.counters ? regen_grid<K, true>() : regen_grid<K, false>();
+    if (grid == 0) return set_error(NB_ERR_HIP, "render: device attribute or occupancy query failed");
+    // every wave's first grab_units units of its XCD's band are assigned statically (no atomic
+    // at the start); the counters start after them
+    const uint32_t first = (grid / 8u) * p.grab_units;
+    hipLaunchKernelGGL(reset_work_kernel, dim3(1), dim3(64), 0, s, p.work, first);
     if (p.counters) hipLaunchKernelGGL((render_pt_kernel<K, true>), dim3(grid), dim3(64), 0, s, p);
     else hipLaunchKernelGGL((render_pt_kernel<K, false>), dim3(grid), dim3(64), 0, s, p);
     hipError_t e = hipGetLastError();
