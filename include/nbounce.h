@@ -278,13 +278,23 @@ nb_status nb_server_stats_get(nb_server *srv, nb_server_stats *out);
 typedef struct nb_cache nb_cache;
 
 uint32_t nb_cache_param_count(void);
+/* BF16 operands (no loss scaling). */
 nb_status nb_cache_create(nb_context *ctx, const float scene_min[3], const float scene_max[3], uint32_t seed,
                           uint32_t max_batch, nb_cache **out);
+/* flags: NB_CACHE_FP16 selects FP16 operands, trained with a dynamic loss scale kept on the device
+ * (starts at 1024; a step whose gradient is not finite is skipped and halves it; 2000 clean steps
+ * double it, up to 65536). */
+#define NB_CACHE_FP16 1u
+nb_status nb_cache_create_ex(nb_context *ctx, const float scene_min[3], const float scene_max[3], uint32_t seed,
+                             uint32_t max_batch, uint32_t flags, nb_cache **out);
+/* The current loss scale (1 for a BF16 cache); waits for the cache's pending work. */
+nb_status nb_cache_loss_scale(nb_cache *cache, float *scale);
 nb_status nb_cache_destroy(nb_cache *cache);
 nb_status nb_cache_get_params(nb_cache *cache, float *out);
 nb_status nb_cache_set_params(nb_cache *cache, const float *in);
-/* Host arrays in and out (used by tests and offline tools): the bf16 input encoding, the
- * network output, the loss and its gradient (no update), and Adam steps over minibatches. */
+/* Host arrays in and out (used by tests and offline tools): the 16-bit input encoding (bf16 or
+ * fp16 bits), the network output, the loss and its gradient (no update; divided by the loss
+ * scale), and Adam steps over minibatches. */
 nb_status nb_cache_encode(nb_cache *cache, const float *raw, uint32_t n, uint16_t *encoded);
 nb_status nb_cache_infer(nb_cache *cache, const float *raw, uint32_t n, float *rgb);
 nb_status nb_cache_gradient(nb_cache *cache, const float *raw, const float *target, uint32_t n, float *loss,

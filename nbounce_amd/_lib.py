@@ -71,6 +71,8 @@ SIGNATURES = {
     "nb_server_now": [_p, _ptr(_c.c_double)],
     "nb_server_stats_get": [_p, _p],
     "nb_cache_create": [_p, _p, _p, _u32, _u32, _c.POINTER(_p)],
+    "nb_cache_create_ex": [_p, _p, _p, _u32, _u32, _u32, _c.POINTER(_p)],
+    "nb_cache_loss_scale": [_p, _ptr(_f32)],
     "nb_cache_destroy": [_p],
     "nb_cache_get_params": [_p, _p],
     "nb_cache_set_params": [_p, _p],

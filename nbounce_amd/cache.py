@@ -9,12 +9,17 @@ from ._lib import check, lib, ptr
 class RadianceCache:
     """MLP radiance cache bound to a context; scene_min/scene_max normalise positions."""
 
-    def __init__(self, ctx, scene_min, scene_max, seed=0, max_batch=1 << 16):
+    def __init__(self, ctx, scene_min, scene_max, seed=0, max_batch=1 << 16, fp16=False):
         lo = np.ascontiguousarray(scene_min, dtype=np.float32)
         hi = np.ascontiguousarray(scene_max, dtype=np.float32)
         h = ctypes.c_void_p()
-        check(lib.nb_cache_create(ctx.handle, ptr(lo), ptr(hi), seed, max_batch, ctypes.byref(h)))
-        self.handle, self.ctx = h, ctx
+        check(lib.nb_cache_create_ex(ctx.handle, ptr(lo), ptr(hi), seed, max_batch, 1 if fp16 else 0, ctypes.byref(h)))
+        self.handle, self.ctx, self.fp16 = h, ctx, fp16
+
+    def loss_scale(self):
+        s = ctypes.c_float()
+        check(lib.nb_cache_loss_scale(self.handle, ctypes.byref(s)))
+        return s.value
 
     @staticmethod
     def param_count():

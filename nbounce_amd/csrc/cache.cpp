@@ -61,7 +61,9 @@ struct nb_cache {
     uint32_t max_batch = 0;
     float scene_min[3], scene_inv[3];
     float *params = nullptr, *adam_m = nullptr, *adam_v = nullptr, *grad = nullptr;
-    uint16_t *wbf = nullptr;     // bf16 weights, last layer padded
+    bool half = false;           // fp16 operands with a dynamic loss scale (else bf16)
+    uint32_t *scale_state = nullptr;   // fp16: [0] loss scale bits, [1] clean steps, [2] overflow flag
+    uint16_t *wbf = nullptr;     // 16-bit weights, last layer padded
     float *b5 = nullptr;         // last-layer bias padded to 16
     uint32_t adam_t = 0;
     // frame workspace (grows with the target)
@@ -80,45 +82,48 @@ struct nb_cache {
             m.b[L] = L == kMlpLayers - 1 ? b5 : params + kL.b[L];
         }
         for (int d = 0; d < 3; ++d) { m.scene_min[d] = scene_min[d]; m.scene_inv_extent[d] = scene_inv[d]; }
+        m.half = half ? 1u : 0u;
         return m;
     }
 };
 
 namespace nb {
 
-// bf16 weight copies and the padded last-layer bias from the fp32 parameters.
+// 16-bit weight copies and the padded last-layer bias from the fp32 parameters.
 nb_status cache_refresh(nb_cache *c) {
     for (int L = 0; L < kMlpLayers; ++L)
-        C_TRY(mlp_launch_to_bf16(c->params + kL.w[L], c->wbf + kL.wb[L], kL.in[L] * kL.out[L], c->stream));
+        C_TRY(mlp_launch_to16(c->half, c->params + kL.w[L], c->wbf + kL.wb[L], kL.in[L] * kL.out[L], c->stream));
     C_HIP(hipMemcpyAsync(c->b5, c->params + kL.b[kMlpLayers - 1], kMlpOut * sizeof(float), hipMemcpyDeviceToDevice,
                          c->stream));
     return NB_OK;
 }
 
-// Forward and backward over the batch in c->raw / c->target (n rows); gradients into c->grad,
-// mean loss into c->loss.  No parameter update.
+// Forward and backward over the batch in c->raw / c->target (n rows); gradients (times the loss
+// scale of an fp16 cache) into c->grad, mean loss into c->loss.  No parameter update.
 nb_status cache_forward_backward(nb_cache *c, const float *raw, const float *target, uint32_t n) {
     const MlpDev m = c->dev();
     hipStream_t s = c->stream;
     C_TRY(mlp_launch_encode(m, raw, n, c->x, s));
     const uint16_t *in = c->x;
+    const bool h = c->half;
     for (int L = 0; L < 5; ++L) {
-        C_TRY(mlp_launch_dense_fwd(in, kL.in[L], m.w[L], m.b[L], kMlpHidden, n, nullptr, c->h[L], 1, s));
+        C_TRY(mlp_launch_dense_fwd(h, in, kL.in[L], m.w[L], m.b[L], kMlpHidden, n, nullptr, c->h[L], 1, s));
         in = c->h[L];
     }
-    C_TRY(mlp_launch_dense_fwd(c->h[4], kMlpHidden, m.w[5], m.b[5], kMlpOut, n, c->y, nullptr, 0, s));
+    C_TRY(mlp_launch_dense_fwd(h, c->h[4], kMlpHidden, m.w[5], m.b[5], kMlpOut, n, c->y, nullptr, 0, s));
     C_TRY(mlp_launch_fill(c->loss, 1, 0.0f, s));
-    C_TRY(mlp_launch_loss(c->y, target, n, nullptr, c->dy, c->loss, s));
+    const float *scale = c->half ? reinterpret_cast<const float *>(c->scale_state) : nullptr;
+    C_TRY(mlp_launch_loss(h, c->y, target, n, nullptr, c->dy, c->loss, scale, s));
     C_TRY(mlp_launch_fill(c->grad, kMlpParams, 0.0f, s));
     // last layer
-    C_TRY(mlp_launch_dense_bwd_weight(c->dy, kMlpOut, c->h[4], kMlpHidden, n, c->grad + kL.w[5], c->grad + kL.b[5], s));
-    C_TRY(mlp_launch_dense_bwd_data(c->dy, kMlpOutPad, m.w[5], kMlpHidden, n, c->h[4], c->dz[4], s));
+    C_TRY(mlp_launch_dense_bwd_weight(h, c->dy, kMlpOut, c->h[4], kMlpHidden, n, c->grad + kL.w[5], c->grad + kL.b[5], s));
+    C_TRY(mlp_launch_dense_bwd_data(h, c->dy, kMlpOutPad, m.w[5], kMlpHidden, n, c->h[4], c->dz[4], s));
     for (int L = 4; L >= 1; --L) {
-        C_TRY(mlp_launch_dense_bwd_weight(c->dz[L], kMlpHidden, c->h[L - 1], kMlpHidden, n, c->grad + kL.w[L],
+        C_TRY(mlp_launch_dense_bwd_weight(h, c->dz[L], kMlpHidden, c->h[L - 1], kMlpHidden, n, c->grad + kL.w[L],
                                           c->grad + kL.b[L], s));
-        C_TRY(mlp_launch_dense_bwd_data(c->dz[L], kMlpHidden, m.w[L], kMlpHidden, n, c->h[L - 1], c->dz[L - 1], s));
+        C_TRY(mlp_launch_dense_bwd_data(h, c->dz[L], kMlpHidden, m.w[L], kMlpHidden, n, c->h[L - 1], c->dz[L - 1], s));
     }
-    C_TRY(mlp_launch_dense_bwd_weight(c->dz[0], kMlpHidden, c->x, kMlpIn, n, c->grad + kL.w[0], c->grad + kL.b[0], s));
+    C_TRY(mlp_launch_dense_bwd_weight(h, c->dz[0], kMlpHidden, c->x, kMlpIn, n, c->grad + kL.w[0], c->grad + kL.b[0], s));
     return NB_OK;
 }
 
@@ -127,7 +132,8 @@ nb_status cache_adam_step(nb_cache *c, float lr) {
     ++c->adam_t;
     const float c1 = static_cast<float>(1.0 / (1.0 - std::pow(0.9, c->adam_t)));
     const float c2 = static_cast<float>(1.0 / (1.0 - std::pow(0.99, c->adam_t)));
-    C_TRY(mlp_launch_adam(c->params, c->adam_m, c->adam_v, c->grad, kMlpParams, lr, b1, b2, eps, c1, c2, c->stream));
+    C_TRY(mlp_launch_adam(c->params, c->adam_m, c->adam_v, c->grad, kMlpParams, lr, b1, b2, eps, c1, c2,
+                          c->half ? c->scale_state : nullptr, c->stream));
     return cache_refresh(c);
 }
 
@@ -145,14 +151,21 @@ uint32_t nb_cache_param_count(void) { return kMlpParams; }
 
 nb_status nb_cache_create(nb_context *ctx, const float scene_min[3], const float scene_max[3], uint32_t seed,
                           uint32_t max_batch, nb_cache **out) {
+    return nb_cache_create_ex(ctx, scene_min, scene_max, seed, max_batch, 0u, out);
+}
+
+nb_status nb_cache_create_ex(nb_context *ctx, const float scene_min[3], const float scene_max[3], uint32_t seed,
+                             uint32_t max_batch, uint32_t flags, nb_cache **out) {
     if (!ctx || !scene_min || !scene_max || !out) return set_error(NB_ERR_INVALID_ARGUMENT, "null argument");
     *out = nullptr;
+    if (flags & ~NB_CACHE_FP16) return set_error(NB_ERR_INVALID_ARGUMENT, "unknown cache flags %#x", flags);
     if (max_batch < 16 || max_batch > (1u << 24)) return set_error(NB_ERR_INVALID_ARGUMENT, "max_batch must be in 16..2^24");
     for (int d = 0; d < 3; ++d)
         if (!(scene_max[d] > scene_min[d])) return set_error(NB_ERR_INVALID_ARGUMENT, "empty scene box");
     C_HIP(hipSetDevice(ctx->device));
     nb_cache *c = new nb_cache();
     c->ctx = ctx;
+    c->half = (flags & NB_CACHE_FP16) != 0;
     context_add_child(ctx);
     c->max_batch = (max_batch + 15) / 16 * 16;
     for (int d = 0; d < 3; ++d) { c->scene_min[d] = scene_min[d]; c->scene_inv[d] = 1.0f / (scene_max[d] - scene_min[d]); }
@@ -165,11 +178,18 @@ nb_status nb_cache_create(nb_context *ctx, const float scene_min[3], const float
     alloc(&c->raw, B * kMlpRaw * 4); alloc(&c->target, B * 3 * 4); alloc(&c->y, B * 3 * 4); alloc(&c->rgb, B * 3 * 4);
     alloc(&c->loss, 4); alloc(&c->x, B * kMlpIn * 2); alloc(&c->dy, B * kMlpOutPad * 2);
     alloc(&c->counts, 16);
+    alloc(&c->scale_state, 16);
     for (int L = 0; L < 5; ++L) { alloc(&c->h[L], B * kMlpHidden * 2); alloc(&c->dz[L], B * kMlpHidden * 2); }
     if (e == hipSuccess) e = hipMemset(c->wbf, 0, kBf16Weights * 2);
     if (e == hipSuccess) e = hipMemset(c->b5, 0, kMlpOutPad * 4);
     if (e == hipSuccess) e = hipMemset(c->adam_m, 0, kMlpParams * 4);
     if (e == hipSuccess) e = hipMemset(c->adam_v, 0, kMlpParams * 4);
+    if (e == hipSuccess) {
+        const float initial_scale = 1024.0f;   // fp16 loss scale at the start (unused for bf16)
+        uint32_t st[4] = {0u, 0u, 0u, 0u};
+        std::memcpy(&st[0], &initial_scale, 4);
+        e = hipMemcpy(c->scale_state, st, 16, hipMemcpyHostToDevice);
+    }
     if (e != hipSuccess) {
         nb_cache_destroy(c);
         return set_error(NB_ERR_OUT_OF_MEMORY, "cache allocation: %s", hipGetErrorString(e));
@@ -195,7 +215,7 @@ nb_status nb_cache_destroy(nb_cache *c) {
     hipSetDevice(c->ctx->device);
     hipStreamSynchronize(c->stream);
     void *ps[] = {c->params, c->adam_m, c->adam_v, c->grad, c->wbf, c->b5, c->raw, c->target, c->y, c->rgb, c->loss,
-                  c->x, c->dy, c->direct, c->q_raw, c->q_thr, c->q_rgb, c->counts};
+                  c->x, c->dy, c->direct, c->q_raw, c->q_thr, c->q_rgb, c->counts, c->scale_state};
     for (void *p : ps) hipFree(p);
     for (int L = 0; L < 5; ++L) { hipFree(c->h[L]); hipFree(c->dz[L]); }
     nb_context *ctx = c->ctx;
@@ -253,7 +273,21 @@ nb_status nb_cache_gradient(nb_cache *c, const float *raw, const float *target, 
     C_TRY(cache_forward_backward(c, c->raw, c->target, n));
     C_HIP(hipMemcpyAsync(loss, c->loss, 4, hipMemcpyDeviceToHost, c->stream));
     C_HIP(hipMemcpyAsync(grad, c->grad, kMlpParams * 4, hipMemcpyDeviceToHost, c->stream));
+    float scale = 1.0f;
+    if (c->half) C_HIP(hipMemcpyAsync(&scale, c->scale_state, 4, hipMemcpyDeviceToHost, c->stream));
     C_HIP(hipStreamSynchronize(c->stream));
+    if (scale != 1.0f)
+        for (uint32_t i = 0; i < kMlpParams; ++i) grad[i] /= scale;
+    return NB_OK;
+}
+
+nb_status nb_cache_loss_scale(nb_cache *c, float *scale) {
+    if (!c || !scale) return set_error(NB_ERR_INVALID_ARGUMENT, "null argument");
+    *scale = 1.0f;
+    if (!c->half) return NB_OK;
+    C_HIP(hipSetDevice(c->ctx->device));
+    C_HIP(hipStreamSynchronize(c->stream));
+    C_HIP(hipMemcpy(scale, c->scale_state, 4, hipMemcpyDeviceToHost));
     return NB_OK;
 }
 

@@ -1,5 +1,9 @@
-// Radiance-cache MLP kernels for gfx950: BF16 operands on the matrix cores
-// (v_mfma_f32_16x16x16_bf16), FP32 accumulation.
+// Radiance-cache MLP kernels for gfx950: 16-bit operands on the matrix cores, FP32 accumulation.
+// Every kernel is instantiated for BF16 (v_mfma_f32_16x16x16_bf16) and for FP16
+// (v_mfma_f32_16x16x16_f16); the FP16 form trains with a dynamic loss scale kept on the device
+// (the output gradient is multiplied by it before rounding to 16 bits; the FP32 weight gradients
+// are divided by it before Adam; a non-finite gradient skips the step and halves the scale,
+// 2000 clean steps double it).
 //
 // Layout convention: every dense layer is computed transposed, Z^T = W H^T, with the weights
 // W (out x in, row-major) as the MFMA A operand and the activations as the B operand.  For
@@ -8,6 +12,8 @@
 // tile of input features 16m..16m+15 of the next layer, so the fused inference kernel keeps a
 // row's activations in registers from the encoding to the output with no shuffles.
 #include "pt_device.h"
+
+#include <hip/hip_fp16.h>
 
 namespace nb {
 namespace {
@@ -24,8 +30,24 @@ __device__ __forceinline__ uint16_t f2bf(float f) {
 }
 __device__ __forceinline__ float bf2f(uint16_t h) { return __uint_as_float(static_cast<uint32_t>(h) << 16); }
 
+typedef _Float16 h4 __attribute__((ext_vector_type(4)));
+
+// binary32 -> the 16-bit operand format (round to nearest even) and back
+template <bool kHalf>
+__device__ __forceinline__ uint16_t to16(float f) {
+    if constexpr (kHalf) return __half_as_ushort(__float2half_rn(f));
+    else return f2bf(f);
+}
+template <bool kHalf>
+__device__ __forceinline__ float from16(uint16_t h) {
+    if constexpr (kHalf) return __half2float(__ushort_as_half(h));
+    else return bf2f(h);
+}
+
+template <bool kHalf>
 __device__ __forceinline__ f4v mfma(s4 a, s4 b, f4v c) {
-    return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
+    if constexpr (kHalf) return __builtin_amdgcn_mfma_f32_16x16x16f16(__builtin_bit_cast(h4, a), __builtin_bit_cast(h4, b), c, 0, 0, 0);
+    else return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
 }
 
 __device__ __forceinline__ s4 load4(const uint16_t *p) {   // 4 consecutive bf16 (8-byte aligned)
@@ -76,6 +98,7 @@ constexpr int kSH = kMlpHidden + kPad;  // 68
 
 // Fused forward pass: weights staged once per workgroup in LDS (47 KB), each wave64 takes 16
 // rows at a time through all six layers in registers.
+template <bool kHalf>
 __global__ __launch_bounds__(256) void mlp_infer_kernel(MlpDev m, const float *raw, uint32_t n_max, const uint32_t *n_dev,
                                                         float *rgb) {
     const uint32_t n = n_dev ? min(*n_dev, n_max) : n_max;
@@ -111,25 +134,25 @@ __global__ __launch_bounds__(256) void mlp_infer_kernel(MlpDev m, const float *r
         }
         s4 x[6];
         for (int t = 0; t < 6; ++t)
-            for (int i = 0; i < 4; ++i) x[t][i] = static_cast<short>(f2bf(feature(r, 16 * t + 4 * kq + i, m)));
+            for (int i = 0; i < 4; ++i) x[t][i] = static_cast<short>(to16<kHalf>(feature(r, 16 * t + 4 * kq + i, m)));
         s4 h[4];
         for (int mt = 0; mt < 4; ++mt) {
             f4v acc = {0.0f, 0.0f, 0.0f, 0.0f};
-            for (int t = 0; t < 6; ++t) acc = mfma(load4(&sw0[(16 * mt + col) * kS0 + 16 * t + 4 * kq]), x[t], acc);
-            for (int i = 0; i < 4; ++i) h[mt][i] = static_cast<short>(f2bf(fmaxf(acc[i] + sb[16 * mt + 4 * kq + i], 0.0f)));
+            for (int t = 0; t < 6; ++t) acc = mfma<kHalf>(load4(&sw0[(16 * mt + col) * kS0 + 16 * t + 4 * kq]), x[t], acc);
+            for (int i = 0; i < 4; ++i) h[mt][i] = static_cast<short>(to16<kHalf>(fmaxf(acc[i] + sb[16 * mt + 4 * kq + i], 0.0f)));
         }
         for (int L = 0; L < 4; ++L) {
             s4 g[4];
             for (int mt = 0; mt < 4; ++mt) {
                 f4v acc = {0.0f, 0.0f, 0.0f, 0.0f};
-                for (int t = 0; t < 4; ++t) acc = mfma(load4(&swh[L][(16 * mt + col) * kSH + 16 * t + 4 * kq]), h[t], acc);
+                for (int t = 0; t < 4; ++t) acc = mfma<kHalf>(load4(&swh[L][(16 * mt + col) * kSH + 16 * t + 4 * kq]), h[t], acc);
                 const float *bl = sb + (L + 1) * kMlpHidden;
-                for (int i = 0; i < 4; ++i) g[mt][i] = static_cast<short>(f2bf(fmaxf(acc[i] + bl[16 * mt + 4 * kq + i], 0.0f)));
+                for (int i = 0; i < 4; ++i) g[mt][i] = static_cast<short>(to16<kHalf>(fmaxf(acc[i] + bl[16 * mt + 4 * kq + i], 0.0f)));
             }
             for (int mt = 0; mt < 4; ++mt) h[mt] = g[mt];
         }
         f4v acc = {0.0f, 0.0f, 0.0f, 0.0f};
-        for (int t = 0; t < 4; ++t) acc = mfma(load4(&sw5[col * kSH + 16 * t + 4 * kq]), h[t], acc);
+        for (int t = 0; t < 4; ++t) acc = mfma<kHalf>(load4(&sw5[col * kSH + 16 * t + 4 * kq]), h[t], acc);
         if (kq == 0 && row < n) {
             const float *b5 = sb + 5 * kMlpHidden;
             for (int c = 0; c < kMlpOut; ++c) rgb[3 * static_cast<size_t>(row) + c] = acc[c] + b5[c];
@@ -137,6 +160,7 @@ __global__ __launch_bounds__(256) void mlp_infer_kernel(MlpDev m, const float *r
     }
 }
 
+template <bool kHalf>
 __global__ void mlp_encode_kernel(MlpDev m, const float *raw, uint32_t n, uint16_t *x) {
     const size_t total = static_cast<size_t>(n) * kMlpIn;
     for (size_t i = blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x; i < total;
@@ -145,12 +169,13 @@ __global__ void mlp_encode_kernel(MlpDev m, const float *raw, uint32_t n, uint16
         const int f = static_cast<int>(i % kMlpIn);
         float r[kMlpRaw];
         for (int q = 0; q < kMlpRaw; ++q) r[q] = raw[row * kMlpRaw + q];
-        x[i] = f2bf(feature(r, f, m));
+        x[i] = to16<kHalf>(feature(r, f, m));
     }
 }
 
 // Z = H_in W^T + b for 16-row tiles; W has round_up(out, 16) rows (padding rows zero).
-// Writes z (fp32, n x out) and/or h_out (bf16, n x out, ReLU applied when relu != 0).
+// Writes z (fp32, n x out) and/or h_out (16-bit, n x out, ReLU applied when relu != 0).
+template <bool kHalf>
 __global__ __launch_bounds__(256) void dense_fwd_kernel(const uint16_t *h_in, uint32_t k_in, const uint16_t *w,
                                                         const float *b, uint32_t out, uint32_t n, float *z,
                                                         uint16_t *h_out, int relu) {
@@ -164,7 +189,7 @@ __global__ __launch_bounds__(256) void dense_fwd_kernel(const uint16_t *h_in, ui
             f4v acc = {0.0f, 0.0f, 0.0f, 0.0f};
             for (uint32_t t = 0; t < k_tiles; ++t) {
                 s4 bop = valid ? load4(h_in + static_cast<size_t>(row) * k_in + 16 * t + 4 * kq) : s4{0, 0, 0, 0};
-                acc = mfma(load4(w + static_cast<size_t>(16 * mt + col) * k_in + 16 * t + 4 * kq), bop, acc);
+                acc = mfma<kHalf>(load4(w + static_cast<size_t>(16 * mt + col) * k_in + 16 * t + 4 * kq), bop, acc);
             }
             if (!valid) continue;
             for (int i = 0; i < 4; ++i) {
@@ -172,19 +197,22 @@ __global__ __launch_bounds__(256) void dense_fwd_kernel(const uint16_t *h_in, ui
                 if (o >= out) continue;
                 const float v = acc[i] + b[o];
                 if (z) z[static_cast<size_t>(row) * out + o] = v;
-                if (h_out) h_out[static_cast<size_t>(row) * out + o] = f2bf(relu ? fmaxf(v, 0.0f) : v);
+                if (h_out) h_out[static_cast<size_t>(row) * out + o] = to16<kHalf>(relu ? fmaxf(v, 0.0f) : v);
             }
         }
     }
 }
 
 // Relative L2 loss (y - t)^2 / (y^2 + 0.01) averaged over rows and channels, with y in the
-// denominator held constant; dy = d loss / d y, also as bf16 padded to 16 channels.
+// denominator held constant; dy = d loss / d y times the loss scale (1 without one), also as
+// 16-bit values padded to 16 channels.
+template <bool kHalf>
 __global__ __launch_bounds__(256) void loss_kernel(const float *y, const float *t, uint32_t n, float *dy,
-                                                   uint16_t *dy_bf, float *loss) {
+                                                   uint16_t *dy_bf, float *loss, const float *loss_scale) {
     __shared__ float part[256];
     float acc = 0.0f;
     const float scale = 1.0f / (3.0f * static_cast<float>(n));
+    const float ls = loss_scale ? *loss_scale : 1.0f;
     for (uint32_t row = blockIdx.x * blockDim.x + threadIdx.x; row < n; row += gridDim.x * blockDim.x) {
         for (int c = 0; c < 16; ++c) {
             float g = 0.0f;
@@ -193,10 +221,10 @@ __global__ __launch_bounds__(256) void loss_kernel(const float *y, const float *
                 const float den = yv * yv + 0.01f;
                 const float d = yv - tv;
                 acc += d * d / den;
-                g = 2.0f * d / den * scale;
+                g = 2.0f * d / den * scale * ls;
                 if (dy) dy[3 * static_cast<size_t>(row) + c] = g;
             }
-            dy_bf[16 * static_cast<size_t>(row) + c] = f2bf(g);
+            dy_bf[16 * static_cast<size_t>(row) + c] = to16<kHalf>(g);
         }
     }
     part[threadIdx.x] = acc;
@@ -208,7 +236,8 @@ __global__ __launch_bounds__(256) void loss_kernel(const float *y, const float *
     if (threadIdx.x == 0) atomicAdd(loss, part[0] * scale);
 }
 
-// dH_prev^T = W^T dZ^T, masked by ReLU'(h_prev): dz (n x out_p bf16), w (out_p x k_in bf16).
+// dH_prev^T = W^T dZ^T, masked by ReLU'(h_prev): dz (n x out_p 16-bit), w (out_p x k_in 16-bit).
+template <bool kHalf>
 __global__ __launch_bounds__(256) void dense_bwd_data_kernel(const uint16_t *dz, uint32_t out_p, const uint16_t *w,
                                                              uint32_t k_in, uint32_t n, const uint16_t *h_prev,
                                                              uint16_t *dz_prev) {
@@ -223,12 +252,12 @@ __global__ __launch_bounds__(256) void dense_bwd_data_kernel(const uint16_t *dz,
             for (uint32_t t = 0; t < k_tiles; ++t) {
                 s4 a = load4_strided(w + static_cast<size_t>(16 * t + 4 * kq) * k_in + 16 * mt + col, k_in);
                 s4 bop = valid ? load4(dz + static_cast<size_t>(row) * out_p + 16 * t + 4 * kq) : s4{0, 0, 0, 0};
-                acc = mfma(a, bop, acc);
+                acc = mfma<kHalf>(a, bop, acc);
             }
             if (!valid) continue;
             for (int i = 0; i < 4; ++i) {
                 const size_t idx = static_cast<size_t>(row) * k_in + 16 * mt + 4 * kq + i;
-                dz_prev[idx] = f2bf(bf2f(h_prev[idx]) > 0.0f ? acc[i] : 0.0f);
+                dz_prev[idx] = to16<kHalf>(from16<kHalf>(h_prev[idx]) > 0.0f ? acc[i] : 0.0f);
             }
         }
     }
@@ -236,6 +265,7 @@ __global__ __launch_bounds__(256) void dense_bwd_data_kernel(const uint16_t *dz,
 
 // dW += dZ^T H_prev over rows, db += column sums of dZ: one (16-output x 16-input) tile per wave
 // over a chunk of rows, then fp32 atomics.  Only outputs < out_valid are written.
+template <bool kHalf>
 __global__ __launch_bounds__(256) void dense_bwd_weight_kernel(const uint16_t *dz, uint32_t out_p, uint32_t out_valid,
                                                                const uint16_t *h_prev, uint32_t k_in, uint32_t n,
                                                                uint32_t rows_per_chunk, float *dw, float *db) {
@@ -257,9 +287,9 @@ __global__ __launch_bounds__(256) void dense_bwd_weight_kernel(const uint16_t *d
                 const bool ok = row < r1;
                 a[i] = ok ? static_cast<short>(dz[static_cast<size_t>(row) * out_p + 16 * mt + col]) : short(0);
                 bop[i] = ok ? static_cast<short>(h_prev[static_cast<size_t>(row) * k_in + 16 * nt + col]) : short(0);
-                if (nt == 0 && ok) bsum += bf2f(static_cast<uint16_t>(a[i]));
+                if (nt == 0 && ok) bsum += from16<kHalf>(static_cast<uint16_t>(a[i]));
             }
-            acc = mfma(a, bop, acc);
+            acc = mfma<kHalf>(a, bop, acc);
         }
         for (int i = 0; i < 4; ++i) {
             const uint32_t o = 16 * mt + 4 * kq + i;
@@ -275,10 +305,14 @@ __global__ __launch_bounds__(256) void dense_bwd_weight_kernel(const uint16_t *d
     }
 }
 
+// One Adam step on the gradient g / loss scale; skipped when the gradient check found a
+// non-finite value (loss-scaled training only: scale == nullptr means no scaling).
 __global__ void adam_kernel(float *w, float *m, float *v, const float *g, uint32_t n, float lr, float b1, float b2,
-                            float eps, float c1, float c2) {
+                            float eps, float c1, float c2, const float *scale, const uint32_t *overflow) {
+    if (overflow && *overflow) return;
+    const float inv = scale ? 1.0f / *scale : 1.0f;
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-        const float gi = g[i];
+        const float gi = g[i] * inv;
         const float mi = b1 * m[i] + (1.0f - b1) * gi;
         const float vi = b2 * v[i] + (1.0f - b2) * gi * gi;
         m[i] = mi;
@@ -287,8 +321,32 @@ __global__ void adam_kernel(float *w, float *m, float *v, const float *g, uint32
     }
 }
 
-__global__ void to_bf16_kernel(const float *src, uint16_t *dst, uint32_t n) {
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) dst[i] = f2bf(src[i]);
+template <bool kHalf>
+__global__ void to16_kernel(const float *src, uint16_t *dst, uint32_t n) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) dst[i] = to16<kHalf>(src[i]);
+}
+
+// Loss-scale state: [0] scale (float bits), [1] clean steps since the last change, [2] overflow
+// flag of the current step.  Sets the flag when any gradient value is not finite.
+__global__ void grad_check_kernel(const float *g, uint32_t n, uint32_t *state) {
+    bool bad = false;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) bad |= !isfinite(g[i]);
+    if (__ballot(bad) != 0ull && (threadIdx.x & 63u) == 0) atomicOr(state + 2, 1u);
+}
+
+// After the step: halve the scale on overflow, double it after 2000 clean steps; clear the flag.
+__global__ void scale_update_kernel(uint32_t *state) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    float s = __uint_as_float(state[0]);
+    if (state[2]) {
+        s = fmaxf(s * 0.5f, 1.0f);
+        state[1] = 0u;
+    } else if (++state[1] >= 2000u) {
+        s = fminf(s * 2.0f, 65536.0f);
+        state[1] = 0u;
+    }
+    state[0] = __float_as_uint(s);
+    state[2] = 0u;
 }
 
 __global__ void fill_kernel(float *p, uint32_t n, float v) {
@@ -308,60 +366,76 @@ uint32_t blocks_for(size_t items, uint32_t per_block, uint32_t cap = 4096) {
 
 }  // namespace
 
+#define NB_MLP_LAUNCH(kernel, grid, block, half, ...)                                              \
+    do {                                                                                          \
+        if (half) hipLaunchKernelGGL(kernel<true>, dim3(grid), dim3(block), 0, s, __VA_ARGS__);  \
+        else hipLaunchKernelGGL(kernel<false>, dim3(grid), dim3(block), 0, s, __VA_ARGS__);      \
+    } while (0)
+
 nb_status mlp_launch_encode(const MlpDev &m, const float *raw, uint32_t n, uint16_t *x, hipStream_t s) {
-    hipLaunchKernelGGL(mlp_encode_kernel, dim3(blocks_for(static_cast<size_t>(n) * kMlpIn, 256)), dim3(256), 0, s, m, raw, n, x);
+    NB_MLP_LAUNCH(mlp_encode_kernel, blocks_for(static_cast<size_t>(n) * kMlpIn, 256), 256, m.half, m, raw, n, x);
     return launched("mlp encode");
 }
 
 nb_status mlp_launch_infer(const MlpDev &m, const float *raw, uint32_t n, float *rgb, hipStream_t s) {
-    hipLaunchKernelGGL(mlp_infer_kernel, dim3(blocks_for(n, 64, 768)), dim3(256), 0, s, m, raw, n,
-                       static_cast<const uint32_t *>(nullptr), rgb);
+    NB_MLP_LAUNCH(mlp_infer_kernel, blocks_for(n, 64, 768), 256, m.half, m, raw, n,
+                  static_cast<const uint32_t *>(nullptr), rgb);
     return launched("mlp infer");
 }
 
 nb_status mlp_launch_infer_count(const MlpDev &m, const float *raw, const uint32_t *n_dev, uint32_t n_max, float *rgb,
                                  hipStream_t s) {
-    hipLaunchKernelGGL(mlp_infer_kernel, dim3(blocks_for(n_max, 64, 768)), dim3(256), 0, s, m, raw, n_max, n_dev, rgb);
+    NB_MLP_LAUNCH(mlp_infer_kernel, blocks_for(n_max, 64, 768), 256, m.half, m, raw, n_max, n_dev, rgb);
     return launched("mlp infer");
 }
 
-nb_status mlp_launch_dense_fwd(const uint16_t *h_in, uint32_t k_in, const uint16_t *w, const float *b, uint32_t out,
-                               uint32_t n, float *z, uint16_t *h_out, int relu, hipStream_t s) {
-    hipLaunchKernelGGL(dense_fwd_kernel, dim3(blocks_for(n, 64)), dim3(256), 0, s, h_in, k_in, w, b, out, n, z, h_out, relu);
+nb_status mlp_launch_dense_fwd(bool half, const uint16_t *h_in, uint32_t k_in, const uint16_t *w, const float *b,
+                               uint32_t out, uint32_t n, float *z, uint16_t *h_out, int relu, hipStream_t s) {
+    NB_MLP_LAUNCH(dense_fwd_kernel, blocks_for(n, 64), 256, half, h_in, k_in, w, b, out, n, z, h_out, relu);
     return launched("dense forward");
 }
 
-nb_status mlp_launch_loss(const float *y, const float *t, uint32_t n, float *dy, uint16_t *dy_bf, float *loss,
-                          hipStream_t s) {
-    hipLaunchKernelGGL(loss_kernel, dim3(blocks_for(n, 256, 1024)), dim3(256), 0, s, y, t, n, dy, dy_bf, loss);
+nb_status mlp_launch_loss(bool half, const float *y, const float *t, uint32_t n, float *dy, uint16_t *dy_bf,
+                          float *loss, const float *loss_scale, hipStream_t s) {
+    NB_MLP_LAUNCH(loss_kernel, blocks_for(n, 256, 1024), 256, half, y, t, n, dy, dy_bf, loss, loss_scale);
     return launched("loss");
 }
 
-nb_status mlp_launch_dense_bwd_data(const uint16_t *dz, uint32_t out_p, const uint16_t *w, uint32_t k_in, uint32_t n,
-                                    const uint16_t *h_prev, uint16_t *dz_prev, hipStream_t s) {
-    hipLaunchKernelGGL(dense_bwd_data_kernel, dim3(blocks_for(n, 64)), dim3(256), 0, s, dz, out_p, w, k_in, n, h_prev, dz_prev);
+nb_status mlp_launch_dense_bwd_data(bool half, const uint16_t *dz, uint32_t out_p, const uint16_t *w, uint32_t k_in,
+                                    uint32_t n, const uint16_t *h_prev, uint16_t *dz_prev, hipStream_t s) {
+    NB_MLP_LAUNCH(dense_bwd_data_kernel, blocks_for(n, 64), 256, half, dz, out_p, w, k_in, n, h_prev, dz_prev);
     return launched("dense backward (data)");
 }
 
-nb_status mlp_launch_dense_bwd_weight(const uint16_t *dz, uint32_t out, const uint16_t *h_prev, uint32_t k_in,
-                                      uint32_t n, float *dw, float *db, hipStream_t s) {
+nb_status mlp_launch_dense_bwd_weight(bool half, const uint16_t *dz, uint32_t out, const uint16_t *h_prev,
+                                      uint32_t k_in, uint32_t n, float *dw, float *db, hipStream_t s) {
     const uint32_t out_p = (out + 15) / 16 * 16;
     const uint32_t chunk = 1024;
     const size_t jobs = static_cast<size_t>(out_p / 16) * (k_in / 16) * ((n + chunk - 1) / chunk);
-    hipLaunchKernelGGL(dense_bwd_weight_kernel, dim3(blocks_for(jobs, 4)), dim3(256), 0, s, dz, out_p, out, h_prev, k_in,
-                       n, chunk, dw, db);
+    NB_MLP_LAUNCH(dense_bwd_weight_kernel, blocks_for(jobs, 4), 256, half, dz, out_p, out, h_prev, k_in, n, chunk, dw, db);
     return launched("dense backward (weights)");
 }
 
 nb_status mlp_launch_adam(float *w, float *m, float *v, const float *g, uint32_t n, float lr, float b1, float b2,
-                          float eps, float c1, float c2, hipStream_t s) {
-    hipLaunchKernelGGL(adam_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, s, w, m, v, g, n, lr, b1, b2, eps, c1, c2);
-    return launched("adam");
+                          float eps, float c1, float c2, uint32_t *scale_state, hipStream_t s) {
+    const float *scale = scale_state ? reinterpret_cast<const float *>(scale_state) : nullptr;
+    const uint32_t *overflow = scale_state ? scale_state + 2 : nullptr;
+    if (scale_state) {
+        hipLaunchKernelGGL(grad_check_kernel, dim3(blocks_for(n, 256, 256)), dim3(256), 0, s, g, n, scale_state);
+        nb_status st = launched("gradient check");
+        if (st != NB_OK) return st;
+    }
+    hipLaunchKernelGGL(adam_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, s, w, m, v, g, n, lr, b1, b2, eps, c1, c2,
+                       scale, overflow);
+    nb_status st = launched("adam");
+    if (st != NB_OK || !scale_state) return st;
+    hipLaunchKernelGGL(scale_update_kernel, dim3(1), dim3(64), 0, s, scale_state);
+    return launched("loss-scale update");
 }
 
-nb_status mlp_launch_to_bf16(const float *src, uint16_t *dst, uint32_t n, hipStream_t s) {
-    hipLaunchKernelGGL(to_bf16_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, s, src, dst, n);
-    return launched("to bf16");
+nb_status mlp_launch_to16(bool half, const float *src, uint16_t *dst, uint32_t n, hipStream_t s) {
+    NB_MLP_LAUNCH(to16_kernel, blocks_for(n, 256), 256, half, src, dst, n);
+    return launched("to 16-bit");
 }
 
 nb_status mlp_launch_fill(float *p, uint32_t n, float v, hipStream_t s) {

@@ -210,9 +210,10 @@ constexpr int kMlpBiases = 5 * kMlpHidden + kMlpOut;
 constexpr int kMlpParams = kMlpWeights + kMlpBiases;
 
 struct MlpDev {
-    const uint16_t *w[kMlpLayers];   // bf16, (out x in) row-major; layer 5 padded to 16 rows
+    const uint16_t *w[kMlpLayers];   // 16-bit (bf16, or fp16 when half), (out x in) row-major; layer 5 padded to 16 rows
     const float *b[kMlpLayers];      // fp32 biases (layer 5 padded to 16)
     float scene_min[3], scene_inv_extent[3];
+    uint32_t half;                   // operands are fp16 (else bf16)
 };
 
 // One frame of cache-assisted rendering plus the training paths traced beside it.
@@ -245,16 +246,18 @@ nb_status mlp_launch_infer_count(const MlpDev &m, const float *raw, const uint32
                                  hipStream_t s);
 nb_status mlp_launch_encode(const MlpDev &m, const float *raw, uint32_t n, uint16_t *x, hipStream_t s);
 nb_status mlp_launch_infer(const MlpDev &m, const float *raw, uint32_t n, float *rgb, hipStream_t s);
-nb_status mlp_launch_dense_fwd(const uint16_t *h_in, uint32_t k_in, const uint16_t *w, const float *b, uint32_t out,
-                               uint32_t n, float *z, uint16_t *h_out, int relu, hipStream_t s);
-nb_status mlp_launch_loss(const float *y, const float *t, uint32_t n, float *dy, uint16_t *dy_bf, float *loss,
-                          hipStream_t s);
-nb_status mlp_launch_dense_bwd_data(const uint16_t *dz, uint32_t out, const uint16_t *w, uint32_t k_in, uint32_t n,
-                                    const uint16_t *h_prev, uint16_t *dz_prev, hipStream_t s);
-nb_status mlp_launch_dense_bwd_weight(const uint16_t *dz, uint32_t out, const uint16_t *h_prev, uint32_t k_in,
-                                      uint32_t n, float *dw, float *db, hipStream_t s);
+nb_status mlp_launch_dense_fwd(bool half, const uint16_t *h_in, uint32_t k_in, const uint16_t *w, const float *b,
+                               uint32_t out, uint32_t n, float *z, uint16_t *h_out, int relu, hipStream_t s);
+nb_status mlp_launch_loss(bool half, const float *y, const float *t, uint32_t n, float *dy, uint16_t *dy_bf,
+                          float *loss, const float *loss_scale, hipStream_t s);
+nb_status mlp_launch_dense_bwd_data(bool half, const uint16_t *dz, uint32_t out, const uint16_t *w, uint32_t k_in,
+                                    uint32_t n, const uint16_t *h_prev, uint16_t *dz_prev, hipStream_t s);
+nb_status mlp_launch_dense_bwd_weight(bool half, const uint16_t *dz, uint32_t out, const uint16_t *h_prev,
+                                      uint32_t k_in, uint32_t n, float *dw, float *db, hipStream_t s);
+// scale_state: the loss-scale state of an fp16 cache ([0] scale bits, [1] clean steps,
+// [2] overflow flag), or null for bf16 (no scaling)
 nb_status mlp_launch_adam(float *w, float *m, float *v, const float *g, uint32_t n, float lr, float b1, float b2,
-                          float eps, float c1, float c2, hipStream_t s);
-nb_status mlp_launch_to_bf16(const float *src, uint16_t *dst, uint32_t n, hipStream_t s);
+                          float eps, float c1, float c2, uint32_t *scale_state, hipStream_t s);
+nb_status mlp_launch_to16(bool half, const float *src, uint16_t *dst, uint32_t n, hipStream_t s);
 nb_status mlp_launch_fill(float *p, uint32_t n, float v, hipStream_t s);
 }  // namespace nb

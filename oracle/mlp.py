@@ -1,10 +1,12 @@
 """CPU restatement of the radiance-cache MLP (nbounce_amd/csrc/mlp.hip, cache.cpp) in float64.
 
-TEST INFRASTRUCTURE ONLY (the checker of the GPU kernels).  `bf16=True` rounds every value the
+TEST INFRASTRUCTURE ONLY (the checker of the GPU kernels).  `bf=True` rounds every value the
 GPU feeds to the matrix cores to bfloat16 (round to nearest even) at the same places: the encoded
 input, each hidden activation, the output gradient and each back-propagated gradient; the
-rounding is treated as the identity in the backward pass, as on the GPU.  With `bf16=False`
-it is the exact float64 network, which the tests check against PyTorch autograd.
+rounding is treated as the identity in the backward pass, as on the GPU.  `fmt="fp16"` rounds to
+IEEE binary16 instead, with the output gradient multiplied by the loss scale before rounding and
+the weight gradients divided by it at the end, as the FP16 kernels do.  With `bf=False` and no
+`fmt` it is the exact float64 network, which the tests check against PyTorch autograd.
 """
 import math
 
@@ -42,6 +44,20 @@ def bf16(x):
     return (u.astype(np.uint32) << 16).view(np.float32).astype(np.float64)
 
 
+def fp16(x):
+    """Round to IEEE binary16 (nearest even) and return as float64."""
+    with np.errstate(over="ignore"):
+        return np.asarray(x, dtype=np.float32).astype(np.float16).astype(np.float64)
+
+
+def _rounding(bf, fmt):
+    if fmt == "fp16":
+        return fp16
+    if fmt == "bf16" or (fmt is None and bf):
+        return bf16
+    return lambda v: np.asarray(v, np.float64)
+
+
 def encode(raw, scene_min, scene_max):
     """The 96 input features of each raw record (see mlp.hip `feature`)."""
     raw = np.asarray(raw, dtype=np.float32)
@@ -67,9 +83,9 @@ def encode(raw, scene_min, scene_max):
     return x
 
 
-def forward(params, x, bf=True):
+def forward(params, x, bf=True, fmt=None):
     """Returns (y, activations) with activations[0] = the (rounded) input."""
-    r = bf16 if bf else (lambda v: np.asarray(v, np.float64))
+    r = _rounding(bf, fmt)
     layers = unpack(params)
     ws = [(r(w), b) for w, b in layers]
     h = [r(x)]
@@ -81,22 +97,23 @@ def forward(params, x, bf=True):
     return y, h, ws
 
 
-def loss_and_grad(params, x, target, bf=True):
+def loss_and_grad(params, x, target, bf=True, fmt=None, scale=1.0):
     """Relative L2 loss mean((y - t)^2 / (y^2 + 0.01)) with y held constant in the denominator,
-    and its gradient with respect to the flat parameter vector."""
-    r = bf16 if bf else (lambda v: np.asarray(v, np.float64))
-    y, h, ws = forward(params, x, bf)
+    and its gradient with respect to the flat parameter vector (back-propagated times `scale`,
+    returned divided by it)."""
+    r = _rounding(bf, fmt)
+    y, h, ws = forward(params, x, bf, fmt)
     n = len(x)
     den = y * y + 0.01
     loss = float(np.mean((y - target) ** 2 / den))
-    dz = r(2.0 * (y - target) / den / (3.0 * n))
+    dz = r(2.0 * (y - target) / den / (3.0 * n) * scale)
     grads = [None] * LAYERS
     for L in range(LAYERS - 1, -1, -1):
         w, _ = ws[L]
         grads[L] = (dz.T @ h[L], dz.sum(axis=0))
         if L > 0:
             dz = r((dz @ w) * (h[L] > 0))
-    return loss, pack(grads), y
+    return loss, pack(grads) / scale, y
 
 
 def init_params(seed=0):

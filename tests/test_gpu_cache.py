@@ -1,5 +1,6 @@
-"""GPU tests of the radiance-cache MLP kernels (BF16 MFMA, FP32 accumulation) against the
-float64 oracle (oracle/mlp.py) that rounds the same values to bfloat16."""
+"""GPU tests of the radiance-cache MLP kernels (BF16 or FP16 MFMA, FP32 accumulation) against the
+float64 oracle (oracle/mlp.py) that rounds the same values to the same 16-bit format (FP16 with
+the cache's loss scale)."""
 import ctypes
 
 import numpy as np
@@ -26,27 +27,38 @@ def _batch(rng, n):
     return raw, target
 
 
-@pytest.fixture(scope="module")
-def cache():
+@pytest.fixture(scope="module", params=["bf16", "fp16"])
+def cache(request):
     import nbounce_amd as nb
     from nbounce_amd._lib import check, lib, ptr
     ctx = nb.Context(0)
     h = ctypes.c_void_p()
-    check(lib.nb_cache_create(ctx.handle, ptr(BOX_MIN), ptr(BOX_MAX), 7, 1 << 16, ctypes.byref(h)))
-    yield lib, ptr, check, h
+    flags = 1 if request.param == "fp16" else 0
+    check(lib.nb_cache_create_ex(ctx.handle, ptr(BOX_MIN), ptr(BOX_MAX), 7, 1 << 16, flags, ctypes.byref(h)))
+    yield lib, ptr, check, h, request.param
     lib.nb_cache_destroy(h)
     ctx.close()
 
 
+def _fmt(cache):
+    return cache[4]
+
+
+def _round(cache, x):
+    return mlp.fp16(x) if _fmt(cache) == "fp16" else mlp.bf16(x)
+
+
 def _encode(cache, raw):
-    lib, ptr, check, h = cache
+    lib, ptr, check, h = cache[:4]
     out = np.zeros((len(raw), 96), dtype=np.uint16)
     check(lib.nb_cache_encode(h, ptr(raw), len(raw), ptr(out)))
+    if _fmt(cache) == "fp16":
+        return out.view(np.float16).astype(np.float64)
     return (out.astype(np.uint32) << 16).view(np.float32).astype(np.float64)
 
 
 def _params(cache):
-    lib, ptr, check, h = cache
+    lib, ptr, check, h = cache[:4]
     p = np.zeros(lib.nb_cache_param_count(), dtype=np.float32)
     check(lib.nb_cache_get_params(h, ptr(p)))
     return p
@@ -56,14 +68,14 @@ def test_encoding_matches_oracle(cache):
     print("case: MLP input encoding, 4096 records", flush=True)
     raw, _ = _batch(np.random.default_rng(0), 4096)
     got = _encode(cache, raw)
-    want = mlp.bf16(mlp.encode(raw, BOX_MIN, BOX_MAX))
-    ulp = np.abs(want) * 2.0 ** -7 + 1e-30
+    want = _round(cache, mlp.encode(raw, BOX_MIN, BOX_MAX))
+    ulp = np.abs(want) * (2.0 ** -10 if _fmt(cache) == "fp16" else 2.0 ** -7) + 1e-30
     assert np.mean(got == want) > 0.995
     assert np.all(np.abs(got - want) <= ulp + 1e-7)
 
 
 def test_fused_inference_matches_oracle(cache):
-    lib, ptr, check, h = cache
+    lib, ptr, check, h = cache[:4]
     print("case: fused MFMA inference, 5000 rows", flush=True)
     raw, _ = _batch(np.random.default_rng(1), 5000)
     params = mlp.init_params(3)
@@ -72,13 +84,13 @@ def test_fused_inference_matches_oracle(cache):
     got = np.zeros((5000, 3), dtype=np.float32)
     check(lib.nb_cache_infer(h, ptr(raw), 5000, ptr(got)))
     x = _encode(cache, raw)
-    want, _, _ = mlp.forward(params.astype(np.float64), x, bf=True)
+    want, _, _ = mlp.forward(params.astype(np.float64), x, fmt=_fmt(cache))
     rel = np.abs(got - want) / (np.abs(want) + 1e-2)
     assert np.median(rel) < 1e-4 and rel.max() < 2e-2
 
 
 def test_gradient_matches_oracle(cache):
-    lib, ptr, check, h = cache
+    lib, ptr, check, h = cache[:4]
     print("case: MFMA forward + backward gradient, 3000 rows", flush=True)
     raw, target = _batch(np.random.default_rng(2), 3000)
     params = mlp.init_params(4)
@@ -88,7 +100,10 @@ def test_gradient_matches_oracle(cache):
     grad = np.zeros(len(params), dtype=np.float32)
     check(lib.nb_cache_gradient(h, ptr(raw), ptr(target), 3000, ctypes.byref(loss), ptr(grad)))
     x = _encode(cache, raw)
-    want_loss, want_grad, _ = mlp.loss_and_grad(params.astype(np.float64), x, target.astype(np.float64), bf=True)
+    scale = ctypes.c_float()
+    check(lib.nb_cache_loss_scale(h, ctypes.byref(scale)))
+    want_loss, want_grad, _ = mlp.loss_and_grad(params.astype(np.float64), x, target.astype(np.float64),
+                                                fmt=_fmt(cache), scale=scale.value)
     assert abs(loss.value - want_loss) <= 1e-4 * want_loss
     off = 0
     for L, (i, o) in enumerate(mlp.DIMS):
@@ -100,7 +115,7 @@ def test_gradient_matches_oracle(cache):
 
 
 def test_training_reduces_the_loss(cache):
-    lib, ptr, check, h = cache
+    lib, ptr, check, h = cache[:4]
     print("case: Adam training on a smooth radiance field", flush=True)
     rng = np.random.default_rng(5)
     raw, _ = _batch(rng, 1 << 15)
@@ -108,7 +123,8 @@ def test_training_reduces_the_loss(cache):
     target = np.stack([0.5 + 0.4 * np.sin(3 * p[:, 0]) * np.cos(2 * p[:, 2]),
                        0.3 + 0.2 * p[:, 1],
                        0.8 * raw[:, 10]], 1).astype(np.float32)
-    check(lib.nb_cache_set_params(h, ptr(mlp.init_params(6))))
+    p0 = mlp.init_params(6)
+    check(lib.nb_cache_set_params(h, ptr(p0)))
     l0 = ctypes.c_float()
     g = np.zeros(lib.nb_cache_param_count(), dtype=np.float32)
     check(lib.nb_cache_gradient(h, ptr(raw[:4096]), ptr(target[:4096]), 4096, ctypes.byref(l0), ptr(g)))
@@ -117,11 +133,15 @@ def test_training_reduces_the_loss(cache):
         check(lib.nb_cache_train(h, ptr(raw), ptr(target), len(raw), 1024, 3e-3, ctypes.byref(last)))
     l1 = ctypes.c_float()
     check(lib.nb_cache_gradient(h, ptr(raw[:4096]), ptr(target[:4096]), 4096, ctypes.byref(l1), ptr(g)))
-    print(f"loss {l0.value:.4f} -> {l1.value:.5f}")
+    scale = ctypes.c_float()
+    check(lib.nb_cache_loss_scale(h, ctypes.byref(scale)))
+    print(f"loss {l0.value:.4f} -> {l1.value:.5f}, loss scale {scale.value}")
     assert l1.value < 0.05 * l0.value
+    assert 1.0 <= scale.value <= 65536.0
 
 
-def test_trained_cache_beats_the_plain_tracer_at_one_sample():
+@pytest.mark.parametrize("fp16", [False, True])
+def test_trained_cache_beats_the_plain_tracer_at_one_sample(fp16):
     """What the trained network gains: at one sample per pixel the cache-assisted image is
     closer to a 1024-spp reference (plain tracer) than the plain 1-spp image, and than the same
     method with an untrained network (which predicts near zero, so its error sits near the relative
@@ -131,7 +151,7 @@ def test_trained_cache_beats_the_plain_tracer_at_one_sample():
     from nbounce_amd import scenegen
     from nbounce_amd.cache import RadianceCache, rel_mse
     from tests import _workloads as Wl
-    print("case: cache-assisted vs plain at 1 spp, 192x112 spheres scene", flush=True)
+    print(f"case: cache-assisted vs plain at 1 spp, 192x112 spheres scene, {'fp16' if fp16 else 'bf16'}", flush=True)
     w_, h_ = 192, 112
     ctx = nb.Context(0)
     wl = Wl.small()
@@ -163,9 +183,9 @@ def test_trained_cache_beats_the_plain_tracer_at_one_sample():
             errs.append(rel_mse(pt.output_texture(), ref))
         return float(np.mean(errs))
 
-    untrained = RadianceCache(ctx, [0, 0, 0], [10, 10, 10], seed=1)
+    untrained = RadianceCache(ctx, [0, 0, 0], [10, 10, 10], seed=1, fp16=fp16)
     e_untrained = cache_err(untrained)
-    cache = RadianceCache(ctx, [0, 0, 0], [10, 10, 10], seed=1)
+    cache = RadianceCache(ctx, [0, 0, 0], [10, 10, 10], seed=1, fp16=fp16)
     train_pt = tracer()
     losses = []
     for f in range(160):

@@ -79,3 +79,33 @@ def test_backprop_matches_finite_differences():
         e[k] = 1e-6
         fd = (loss_at(params + e) - loss_at(params - e)) / 2e-6
         assert abs(fd - grad[k]) <= 1e-6 * max(1.0, abs(fd)) + 1e-9
+
+
+def test_fp16_rounding_closed_forms():
+    """fp16() is IEEE binary16 round-to-nearest-even (ties to the even significand)."""
+    one = 1.0
+    assert mlp.fp16(one + 2.0 ** -11) == one                     # tie -> even (1.0)
+    assert mlp.fp16(one + 3 * 2.0 ** -11) == one + 2.0 ** -9     # tie -> even (next up is odd)
+    assert mlp.fp16(65504.0) == 65504.0 and np.isinf(mlp.fp16(65520.0))
+    assert mlp.fp16(2.0 ** -24) == 2.0 ** -24 and mlp.fp16(2.0 ** -26) == 0.0
+
+
+def test_loss_scale_is_divided_back_out():
+    """Without rounding, back-propagating scale * dL/dy and dividing by scale is the identity."""
+    rng = np.random.default_rng(3)
+    x = rng.normal(size=(40, mlp.IN))
+    t = rng.uniform(0, 2, size=(40, mlp.OUT))
+    p = mlp.init_params(2).astype(np.float64)
+    l1, g1, _ = mlp.loss_and_grad(p, x, t, bf=False)
+    l2, g2, _ = mlp.loss_and_grad(p, x, t, bf=False, scale=1024.0)
+    assert l1 == l2
+    np.testing.assert_array_equal(g1, g2)
+    # a nearly converged batch: dL/dy ~ 1e-8 is below binary16's smallest subnormal (6e-8), so
+    # unscaled fp16 back-propagation loses the gradient and a scale of 1024 keeps it
+    y, _, _ = mlp.forward(p, x, fmt="fp16")
+    near = y * (1.0 + 1e-6) + 1e-7
+    _, g16, _ = mlp.loss_and_grad(p, x, near, fmt="fp16", scale=1024.0)
+    _, g16u, _ = mlp.loss_and_grad(p, x, near, fmt="fp16", scale=1.0)
+    _, g64, _ = mlp.loss_and_grad(p, x, near, fmt="fp16", scale=2.0 ** 20)
+    assert np.count_nonzero(g16u) < 0.5 * np.count_nonzero(g16)
+    assert np.linalg.norm(g16 - g64) < 0.1 * np.linalg.norm(g64)
