@@ -158,6 +158,114 @@ def serve_bench(args):
     print(json.dumps(out), flush=True)
 
 
+def serve_cache_bench(args):
+    """Serving with learned caches: the serving workload (4 resident instanced scenes, Poisson
+    arrivals at --qps, mixed resolution and spp, 8 bounces) where every scene keeps its own
+    device-resident cache and Adam state (scenes 0 and 2 BF16, 1 and 3 FP16 with loss scaling)
+    and each request is plain or cache-assisted with probability 1/2.  A cache-assisted request
+    asks for the samples that reach the plain request's relative MSE: its spp divided by the
+    plain-spp equivalent of one cached sample, measured first on scene 0 against a 1024-spp plain
+    reference.  Cached requests train their scene's cache online.  Reports p50/p99 latency and
+    Mrays/s per mode."""
+    import nbounce_amd as nb
+    from nbounce_amd import scenegen, serving
+    from nbounce_amd.cache import RadianceCache, rel_mse
+    from nbounce_amd._lib import check, lib, ptr
+    ctx = nb.Context(0)
+    table = nb.sobol_table(1024, 17, 0)
+    check(lib.nb_sobol_set(ctx.handle, ptr(table), 1024, 17, 17))
+    srv = serving.Server(ctx, hbm_budget_bytes=200 << 30, max_paths=args.max_paths, max_bounces=12)
+    env = nb.EnvMap(ctx, scenegen.sky_cube(256))
+    keep, caches = [], []
+    for seed, n_inst in zip(range(4), (1024, 2048, 3072, 4096)):
+        v, i, pr = scenegen.instanced_meshes(seed, n_instances=n_inst)
+        sc = nb.Scene(v, i, pr)
+        b, t, inst = nb.build_scene(sc, share_blas=True)
+        sb = nb.SceneBuffers(ctx, b, t, inst, sc.vertices, sc.indices)
+        idx = srv.add_scene(sb, env)
+        cache = RadianceCache(ctx, [0, 0, 0], [200, 200, 200], seed=seed, fp16=bool(seed % 2))
+        srv.set_cache(idx, cache)
+        keep.append(sb)
+        caches.append(cache)
+    reqs = serving.mixed_requests(args.requests, args.qps, 4, seed=0)
+
+    def camera(r):
+        return nb.CameraController(world_position=r["eye"], target=r["target"],
+                                   aspect_ratio=r["width"] / r["height"]).data()
+
+    # warm-up training of every cache: cached frames at 256x256 from request cameras
+    rng = np.random.default_rng(1)
+    for k, (sb, cache) in enumerate(zip(keep, caches)):
+        for f in range(args.cache_warmup):
+            r = reqs[int(rng.integers(0, len(reqs)))]
+            cam = nb.CameraController(world_position=r["eye"], target=r["target"], aspect_ratio=1.0)
+            pt = nb.Pathtracer(ctx, sb, cam, env, 256, 256, resolution_factor=1.0, table=table, sobol_stride=17)
+            cache.render(pt, train_paths=8192, lr=args.lr)
+
+    # equal-error calibration on scene 0 (128x128, a request camera)
+    r0 = next(r for r in reqs if r["scene"] == 0)
+    cam = nb.CameraController(world_position=r0["eye"], target=r0["target"], aspect_ratio=1.0)
+    ref_pt = nb.Pathtracer(ctx, keep[0], cam, env, 128, 128, resolution_factor=1.0, table=table, sobol_stride=17)
+    ref_pt.dispatch_frames(1023)
+    ref = ref_pt.output_texture()
+    plain_err = []
+    for n in (1, 2, 4, 8):
+        pt = nb.Pathtracer(ctx, keep[0], cam, env, 128, 128, resolution_factor=1.0, table=table, sobol_stride=17)
+        pt.dispatch_frames(n)
+        plain_err.append((n, rel_mse(pt.output_texture(), ref)))
+    pt = nb.Pathtracer(ctx, keep[0], cam, env, 128, 128, resolution_factor=1.0, table=table, sobol_stride=17)
+    caches[0].render(pt, train_paths=0)
+    e1 = rel_mse(pt.output_texture(), ref)
+    xs, ys = np.log([p[0] for p in plain_err]), np.log([p[1] for p in plain_err])
+    factor = float(np.exp(np.interp(-np.log(e1), -ys, xs)))
+    factor = max(1.0, factor)
+    for r in reqs:
+        r["cached"] = bool(rng.random() < 0.5)
+        if r["cached"]:
+            r["spp_plain"] = r["spp"]
+            r["spp"] = max(1, int(round(r["spp"] / factor)))
+
+    serving.run_open_loop(srv, [dict(r, arrival=0.0, cached=False) for r in reqs[:2]], camera)
+    s0 = srv.stats()
+    ctx.reset_counters()
+    lat, wall, _ = serving.run_open_loop(srv, reqs, camera, train_paths=args.serve_train_paths, lr=args.lr)
+    s1 = srv.stats()
+    cc = ctx.counters()
+    cached = np.array([r["cached"] for r in reqs])
+    plain_rays = s1["rays"] - s0["rays"]
+    plain_busy = (s1["gpu_busy_s"] - s0["gpu_busy_s"]) - (s1["cached_busy_s"] - s0["cached_busy_s"])
+    cached_busy = s1["cached_busy_s"] - s0["cached_busy_s"]
+
+    def pct(a, q):
+        return round(float(np.percentile(a, q)) * 1e3, 2) if len(a) else None
+
+    out = {
+        "metric": "serving with learned caches: equal-relative-MSE request latency p50/p99 per mode",
+        "value": pct(lat[cached], 99),
+        "unit": "ms (p99 latency, cache-assisted requests)",
+        "higher_is_better": False,
+        "plain": {"p50_ms": pct(lat[~cached], 50), "p99_ms": pct(lat[~cached], 99), "requests": int((~cached).sum()),
+                  "mrays_s_busy": round(plain_rays / max(plain_busy, 1e-9) / 1e6, 2)},
+        "cached": {"p50_ms": pct(lat[cached], 50), "p99_ms": pct(lat[cached], 99), "requests": int(cached.sum()),
+                   "mrays_s_busy": round(cc["rays"] / max(cached_busy, 1e-9) / 1e6, 2),
+                   "samples": int(s1["cached_samples"] - s0["cached_samples"])},
+        "equal_error_plain_spp_per_cached_spp": round(factor, 3),
+        "calibration": {"plain": [[n, round(e, 5)] for n, e in plain_err], "cached_1spp": round(e1, 5)},
+        "loss_scale_fp16": [c.loss_scale() for c in caches if c.fp16],
+        "aggregate_mrays_s": round((plain_rays + cc["rays"]) / wall / 1e6, 2),
+        "gpu_busy_frac": round((s1["gpu_busy_s"] - s0["gpu_busy_s"]) / wall, 3),
+        "requests": args.requests,
+        "qps": args.qps,
+        "wall_s": round(wall, 3),
+        "n_gpus": 1,
+        "config": {"workload": "4 resident instanced scenes (1024-4096 instances) with per-scene caches "
+                               "(BF16, FP16, BF16, FP16), Poisson arrivals, mixed resolution and spp, 8 bounces, "
+                               "half the requests cache-assisted", "cache_warmup_frames": args.cache_warmup,
+                   "serve_train_paths": args.serve_train_paths, "hbm_budget_gb": 200},
+    }
+    print(json.dumps(out), flush=True)
+
+
 def cache_bench(args):
     """Learned radiance cache on the spheres-in-box scene at 1920x1080, 8 bounces: 240 frames of
     one sample while the camera orbits, each frame training the cache on `--train-paths` full
@@ -257,7 +365,10 @@ def main():
     ap.add_argument("--reference-blas", action="store_true", help="rebuild the BLAS per instance as the reference")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
-    ap.add_argument("--mode", choices=["render", "serve", "cache"], default="render")
+    ap.add_argument("--mode", choices=["render", "serve", "cache", "serve-cache"], default="render")
+    ap.add_argument("--cache-warmup", type=int, default=40, help="serve-cache: training frames per scene cache")
+    ap.add_argument("--serve-train-paths", type=int, default=2048,
+                    help="serve-cache: training paths per sample of a cache-assisted request")
     ap.add_argument("--train-paths", type=int, default=21845)
     ap.add_argument("--lr", type=float, default=2e-3)
     ap.add_argument("--qps", type=float, default=20.0)
@@ -276,6 +387,9 @@ def main():
         return
     if args.mode == "cache":
         cache_bench(args)
+        return
+    if args.mode == "serve-cache":
+        serve_cache_bench(args)
         return
 
     import nbounce_amd as nb

@@ -78,6 +78,7 @@ typedef struct nb_context nb_context;
 typedef struct nb_scene nb_scene;
 typedef struct nb_target nb_target;
 typedef struct nb_envmap nb_envmap;
+typedef struct nb_cache nb_cache;
 
 /* ---- errors / library ------------------------------------------------------------------ */
 const char *nb_last_error(void);
@@ -242,21 +243,34 @@ typedef struct {
     uint32_t bounces;
     float contribution_factor;
     double arrival_s;               /* server clock; negative = the time of submission */
+    uint32_t mode;                  /* NB_REQUEST_PLAIN, or NB_REQUEST_CACHED (the scene needs a cache) */
+    uint32_t train_paths;           /* cached: training paths traced per sample (0 = inference only) */
+    float lr;                       /* cached: Adam learning rate of that training */
+    uint32_t pad;
 } nb_request;
+
+#define NB_REQUEST_PLAIN 0u
+#define NB_REQUEST_CACHED 1u
 
 typedef struct { uint64_t id; double arrival_s, admit_s, done_s; } nb_completion;
 
 typedef struct {
-    uint64_t steps, rays, samples, n_aabb, n_tri, n_inst, hits;
+    uint64_t steps, rays, samples, n_aabb, n_tri, n_inst, hits;   /* plain requests' kernels */
     uint64_t bytes_in_use, bytes_budget;
     uint32_t active, queued, completed, pad;
     double gpu_busy_s;              /* summed device time of the steps */
+    uint64_t cached_samples;        /* samples rendered for cached requests */
+    double cached_busy_s;           /* their device time (part of gpu_busy_s) */
 } nb_server_stats;
 
 nb_status nb_server_create(nb_context *ctx, uint64_t hbm_budget_bytes, uint32_t max_paths_per_step,
                            uint32_t max_bounces, nb_server **out);
 nb_status nb_server_destroy(nb_server *srv);
 nb_status nb_server_add_scene(nb_server *srv, const nb_scene *scene, const nb_envmap *env, uint32_t *index);
+/* Gives resident scene `scene` a learned radiance cache (created on the server's context, BF16 or
+ * FP16): NB_REQUEST_CACHED requests for the scene render with nb_render_cached's method on it and
+ * train it online.  The cache is shared by those requests and kept for the server's life. */
+nb_status nb_server_set_cache(nb_server *srv, uint32_t scene, nb_cache *cache);
 nb_status nb_server_submit(nb_server *srv, const nb_request *req, uint64_t *id);
 /* Admits what fits, renders one step of the admitted requests and waits for it. */
 nb_status nb_server_step(nb_server *srv, uint32_t *n_active, uint32_t *n_queued);
@@ -275,8 +289,6 @@ nb_status nb_server_stats_get(nb_server *srv, nb_server_stats *out);
  * Adam on a relative L2 loss.  Raw records are 16 floats: position xyz, outgoing direction xyz,
  * normal xyz, roughness, albedo rgb, metallic, 0, 0.  Parameters are one flat fp32 vector:
  * W0 (64x96), b0, W1..W4 (64x64) with biases, W5 (3x64), b5; W is (out x in) row-major. */
-typedef struct nb_cache nb_cache;
-
 uint32_t nb_cache_param_count(void);
 /* BF16 operands (no loss scaling). */
 nb_status nb_cache_create(nb_context *ctx, const float scene_min[3], const float scene_max[3], uint32_t seed,

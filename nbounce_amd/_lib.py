@@ -63,6 +63,7 @@ SIGNATURES = {
     "nb_server_create": [_p, _c.c_uint64, _u32, _u32, _c.POINTER(_p)],
     "nb_server_destroy": [_p],
     "nb_server_add_scene": [_p, _p, _p, _ptr(_u32)],
+    "nb_server_set_cache": [_p, _u32, _p],
     "nb_server_submit": [_p, _p, _ptr(_c.c_uint64)],
     "nb_server_step": [_p, _ptr(_u32), _ptr(_u32)],
     "nb_server_poll": [_p, _p, _u32, _ptr(_u32)],

@@ -310,17 +310,18 @@ nb_status nb_cache_train(nb_cache *c, const float *raw, const float *target, uin
     return NB_OK;
 }
 
-nb_status nb_render_cached(nb_context *ctx, const nb_scene *scene, nb_target *target, const nb_camera *camera,
-                           const nb_globals *g, const nb_envmap *env, nb_cache *c, uint32_t train_paths, float lr,
-                           float *loss) {
-    if (!ctx || !scene || !target || !camera || !g || !env || !c) return set_error(NB_ERR_INVALID_ARGUMENT, "null argument");
-    if (scene->ctx != ctx || target->ctx != ctx || env->ctx != ctx || c->ctx != ctx)
-        return set_error(NB_ERR_INVALID_ARGUMENT, "all objects must belong to the context");
-    if (!ctx->sobol) return set_error(NB_ERR_INVALID_ARGUMENT, "no Sobol table set (nb_sobol_set)");
-    if (g->sample == 0) return set_error(NB_ERR_INVALID_ARGUMENT, "samples are 1-based");
-    C_HIP(hipSetDevice(ctx->device));
-    const size_t npix = static_cast<size_t>(target->width) * target->height;
+}  // extern "C"
+
+namespace nb {
+
+// One cache-assisted frame into a float4 target of width x height pixels on the context's stream
+// (nb_render_cached's work; also used by the server for cached requests).
+nb_status cache_render_frame(nb_context *ctx, const DevScene &scene, const DevEnv &env, float4 *target, uint32_t width,
+                             uint32_t height, const float *c2w, const nb_globals &g, nb_cache *c, uint32_t train_paths,
+                             float lr, float *loss) {
+    const size_t npix = static_cast<size_t>(width) * height;
     if (npix > c->ws_pixels) {
+        C_HIP(hipStreamSynchronize(ctx->stream));
         hipFree(c->direct); hipFree(c->q_raw); hipFree(c->q_thr); hipFree(c->q_rgb);
         c->direct = nullptr; c->q_raw = c->q_thr = c->q_rgb = nullptr; c->ws_pixels = 0;
         C_HIP(hipMalloc(&c->direct, npix * 16));
@@ -330,37 +331,36 @@ nb_status nb_render_cached(nb_context *ctx, const nb_scene *scene, nb_target *ta
         c->ws_pixels = npix;
     }
     CacheFrame f{};
-    f.scene = scene->dev;
-    f.env = DevEnv{env->texels, env->n};
+    f.scene = scene;
+    f.env = env;
     f.sobol = ctx->sobol;
     f.sobol_entries = ctx->sobol_entries;
     f.sobol_stride = ctx->sobol_stride;
-    std::memcpy(f.c2w, camera->clip_to_world, sizeof(f.c2w));
-    f.width = target->width;
-    f.height = target->height;
-    f.sample = g->sample;
-    f.weight = g->weight;
-    f.bounces = g->bounces;
-    f.contribution_factor = g->contribution_factor;
+    std::memcpy(f.c2w, c2w, sizeof(f.c2w));
+    f.width = width;
+    f.height = height;
+    f.sample = g.sample;
+    f.weight = g.weight;
+    f.bounces = g.bounces;
+    f.contribution_factor = g.contribution_factor;
     f.mirror_alpha = 0.05f;
     f.max_cache_bounce = 3;
-    f.target = target->pixels;
+    f.target = target;
     f.direct = c->direct;
     f.q_raw = c->q_raw;
     f.q_thr = c->q_thr;
     f.q_count = c->counts;
     f.train_paths = train_paths;
-    f.train_offset = static_cast<uint32_t>((static_cast<uint64_t>(g->sample) * 2654435761ull) % npix);
-    f.train_sample = 1u + (g->sample + 511u) % 1023u;
+    f.train_offset = static_cast<uint32_t>((static_cast<uint64_t>(g.sample) * 2654435761ull) % npix);
+    f.train_sample = 1u + (g.sample + 511u) % 1023u;
     f.t_capacity = c->max_batch;
     f.t_raw = c->raw;
     f.t_target = c->target;
     f.t_count = c->counts + 1;
     f.counters = ctx->counting ? ctx->counters : nullptr;
-    f.tiles_x = target->width / 8;
-    f.n_tiles = (target->width / 8) * (target->height / 8);
+    f.tiles_x = width / 8;
+    f.n_tiles = (width / 8) * (height / 8);
     f.tiles_per_xcd = (f.n_tiles + 7) / 8;
-    C_HIP(hipEventRecord(ctx->ev_start, ctx->stream));
     if (train_paths > 0) {
         C_TRY(cache_launch_train_paths(f, ctx->stream));
         uint32_t n = 0;
@@ -375,7 +375,27 @@ nb_status nb_render_cached(nb_context *ctx, const nb_scene *scene, nb_target *ta
         }
         if (loss) C_HIP(hipMemcpyAsync(loss, c->loss, 4, hipMemcpyDeviceToHost, ctx->stream));
     }
-    C_TRY(cache_launch_frame(f, c->dev(), c->q_rgb, ctx->stream));
+    return cache_launch_frame(f, c->dev(), c->q_rgb, ctx->stream);
+}
+
+nb_context *cache_context(const nb_cache *c) { return c->ctx; }
+
+}  // namespace nb
+
+extern "C" {
+
+nb_status nb_render_cached(nb_context *ctx, const nb_scene *scene, nb_target *target, const nb_camera *camera,
+                           const nb_globals *g, const nb_envmap *env, nb_cache *c, uint32_t train_paths, float lr,
+                           float *loss) {
+    if (!ctx || !scene || !target || !camera || !g || !env || !c) return set_error(NB_ERR_INVALID_ARGUMENT, "null argument");
+    if (scene->ctx != ctx || target->ctx != ctx || env->ctx != ctx || c->ctx != ctx)
+        return set_error(NB_ERR_INVALID_ARGUMENT, "all objects must belong to the context");
+    if (!ctx->sobol) return set_error(NB_ERR_INVALID_ARGUMENT, "no Sobol table set (nb_sobol_set)");
+    if (g->sample == 0) return set_error(NB_ERR_INVALID_ARGUMENT, "samples are 1-based");
+    C_HIP(hipSetDevice(ctx->device));
+    C_HIP(hipEventRecord(ctx->ev_start, ctx->stream));
+    C_TRY(cache_render_frame(ctx, scene->dev, DevEnv{env->texels, env->n}, target->pixels, target->width,
+                             target->height, camera->clip_to_world, *g, c, train_paths, lr, loss));
     C_HIP(hipEventRecord(ctx->ev_stop, ctx->stream));
     ctx->timed = true;
     if (loss && train_paths > 0) C_HIP(hipStreamSynchronize(ctx->stream));

@@ -241,6 +241,10 @@ struct CacheFrame {
 };
 
 nb_status cache_launch_frame(const CacheFrame &p, const MlpDev &m, float *rgb, hipStream_t s);
+nb_status cache_render_frame(nb_context *ctx, const DevScene &scene, const DevEnv &env, float4 *target, uint32_t width,
+                             uint32_t height, const float *c2w, const nb_globals &g, nb_cache *c, uint32_t train_paths,
+                             float lr, float *loss);
+nb_context *cache_context(const nb_cache *c);
 nb_status cache_launch_train_paths(const CacheFrame &p, hipStream_t s);
 nb_status mlp_launch_infer_count(const MlpDev &m, const float *raw, const uint32_t *n_dev, uint32_t n_max, float *rgb,
                                  hipStream_t s);

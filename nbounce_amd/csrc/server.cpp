@@ -74,6 +74,9 @@ struct nb_server {
     ServeRequestDev *h_requests = nullptr;
     ServeSceneDev *h_scenes = nullptr;
     uint32_t n_scenes = 0;
+    nb_cache *caches[kMaxScenes] = {};          // per resident scene, for cached requests
+    uint64_t cached_samples = 0;
+    double cached_busy = 0;
 
     hipGraphExec_t graph = nullptr;
     uint64_t next_id = 1;
@@ -222,6 +225,14 @@ nb_status nb_server_add_scene(nb_server *s, const nb_scene *scene, const nb_envm
     return NB_OK;
 }
 
+nb_status nb_server_set_cache(nb_server *s, uint32_t scene, nb_cache *cache) {
+    if (!s || !cache) return set_error(NB_ERR_INVALID_ARGUMENT, "null argument");
+    if (scene >= s->n_scenes) return set_error(NB_ERR_INVALID_ARGUMENT, "scene %u not added", scene);
+    if (cache_context(cache) != s->ctx) return set_error(NB_ERR_INVALID_ARGUMENT, "the cache belongs to another context");
+    s->caches[scene] = cache;
+    return NB_OK;
+}
+
 nb_status nb_server_submit(nb_server *s, const nb_request *req, uint64_t *id) {
     if (!s || !req || !id) return set_error(NB_ERR_INVALID_ARGUMENT, "null argument");
     if (req->scene >= s->n_scenes) return set_error(NB_ERR_INVALID_ARGUMENT, "scene %u not added", req->scene);
@@ -232,6 +243,10 @@ nb_status nb_server_submit(nb_server *s, const nb_request *req, uint64_t *id) {
                                                  static_cast<unsigned long long>(pixels), s->max_paths);
     if (req->spp == 0) return set_error(NB_ERR_INVALID_ARGUMENT, "spp must be >= 1");
     if (req->bounces > s->max_bounces) return set_error(NB_ERR_INVALID_ARGUMENT, "bounces %u > server maximum %u", req->bounces, s->max_bounces);
+    if (req->mode != NB_REQUEST_PLAIN && req->mode != NB_REQUEST_CACHED)
+        return set_error(NB_ERR_INVALID_ARGUMENT, "unknown request mode %u", req->mode);
+    if (req->mode == NB_REQUEST_CACHED && !s->caches[req->scene])
+        return set_error(NB_ERR_INVALID_ARGUMENT, "scene %u has no cache (nb_server_set_cache)", req->scene);
     const uint64_t bytes = pixels * 16;
     if (bytes > s->budget) return set_error(NB_ERR_BUDGET, "request needs %llu B, more than the HBM budget", static_cast<unsigned long long>(bytes));
     Request *r = new Request();
@@ -268,8 +283,8 @@ nb_status nb_server_step(nb_server *s, uint32_t *n_active, uint32_t *n_queued) {
     uint64_t pixel_sum = 0;
     paths = 0;
     for (size_t k = 0; k < s->active.size(); ++k) {
-        if (!take[k]) continue;
         Request *r = s->active[k];
+        if (!take[k] || r->req.mode == NB_REQUEST_CACHED) continue;
         const uint32_t px = r->req.width * r->req.height;
         ServePass &p = s->h_passes[n_passes++];
         p.slot = static_cast<uint32_t>(r->slot);
@@ -281,6 +296,7 @@ nb_status nb_server_step(nb_server *s, uint32_t *n_active, uint32_t *n_queued) {
         paths += static_cast<uint64_t>(take[k]) * px;
         pixel_sum += px;
     }
+    bool any = false;
     if (n_passes > 0) {
         ServeStep &st = *s->h_step;
         st.scenes = s->d_scenes;
@@ -308,6 +324,43 @@ nb_status nb_server_step(nb_server *s, uint32_t *n_active, uint32_t *n_queued) {
         float ms = 0;
         hipEventElapsedTime(&ms, s->ev0, s->ev1);
         s->busy += ms * 1e-3;
+        any = true;
+    }
+    // cached requests: their sample passes, one cache-assisted frame per sample, in admission
+    // order, on the context's stream (the caches' stream)
+    bool cached = false;
+    for (size_t k = 0; k < s->active.size(); ++k) {
+        Request *r = s->active[k];
+        if (!take[k] || r->req.mode != NB_REQUEST_CACHED) continue;
+        nb_context *ctx = s->ctx;
+        if (!cached) {
+            SRV_HIP(hipStreamSynchronize(s->stream));   // admission zeroed the targets on the server stream
+            SRV_HIP(hipEventRecord(ctx->ev_region_start, ctx->stream));
+        }
+        cached = true;
+        const ServeSceneDev &sc = s->h_scenes[r->req.scene];
+        for (uint32_t j = 0; j < take[k]; ++j) {
+            nb_globals g;
+            g.sample = r->next_sample + j;
+            g.weight = 1.0f / static_cast<float>(g.sample);
+            g.bounces = r->req.bounces;
+            g.contribution_factor = r->req.contribution_factor;
+            SRV_TRY(cache_render_frame(ctx, sc.scene, sc.env, r->target, r->req.width, r->req.height,
+                                       r->req.camera.clip_to_world, g, s->caches[r->req.scene], r->req.train_paths,
+                                       r->req.lr, nullptr));
+        }
+        s->cached_samples += static_cast<uint64_t>(take[k]) * r->req.width * r->req.height;
+    }
+    if (cached) {
+        SRV_HIP(hipEventRecord(s->ctx->ev_region_stop, s->ctx->stream));
+        SRV_HIP(hipEventSynchronize(s->ctx->ev_region_stop));
+        float ms = 0;
+        hipEventElapsedTime(&ms, s->ctx->ev_region_start, s->ctx->ev_region_stop);
+        s->busy += ms * 1e-3;
+        s->cached_busy += ms * 1e-3;
+        any = true;
+    }
+    if (any) {
         ++s->steps;
         const double t = s->now();
         for (size_t k = 0; k < s->active.size(); ++k) s->active[k]->next_sample += take[k];
@@ -382,6 +435,8 @@ nb_status nb_server_stats_get(nb_server *s, nb_server_stats *out) {
     out->bytes_in_use = s->in_use;
     out->bytes_budget = s->budget;
     out->active = static_cast<uint32_t>(s->active.size());
+    out->cached_samples = s->cached_samples;
+    out->cached_busy_s = s->cached_busy;
     out->queued = static_cast<uint32_t>(s->queued.size());
     out->completed = s->completed;
     out->gpu_busy_s = s->busy;

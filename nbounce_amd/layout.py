@@ -23,16 +23,17 @@ COUNTERS = np.dtype([("rays", "<u8"), ("n_aabb", "<u8"), ("n_tri", "<u8"), ("n_i
 
 # nb_request / nb_completion / nb_server_stats (include/nbounce.h, serving)
 REQUEST = np.dtype([("camera", CAMERA), ("scene", "<u4"), ("width", "<u4"), ("height", "<u4"), ("spp", "<u4"),
-                    ("bounces", "<u4"), ("contribution_factor", "<f4"), ("arrival_s", "<f8")], align=True)
+                    ("bounces", "<u4"), ("contribution_factor", "<f4"), ("arrival_s", "<f8"), ("mode", "<u4"),
+                    ("train_paths", "<u4"), ("lr", "<f4"), ("pad", "<u4")], align=True)
 COMPLETION = np.dtype([("id", "<u8"), ("arrival_s", "<f8"), ("admit_s", "<f8"), ("done_s", "<f8")])
 SERVER_STATS = np.dtype([("steps", "<u8"), ("rays", "<u8"), ("samples", "<u8"), ("n_aabb", "<u8"), ("n_tri", "<u8"),
                          ("n_inst", "<u8"), ("hits", "<u8"), ("bytes_in_use", "<u8"), ("bytes_budget", "<u8"),
                          ("active", "<u4"), ("queued", "<u4"), ("completed", "<u4"), ("pad", "<u4"),
-                         ("gpu_busy_s", "<f8")])
+                         ("gpu_busy_s", "<f8"), ("cached_samples", "<u8"), ("cached_busy_s", "<f8")])
 
 assert NODE.itemsize == 32 and VERTEX.itemsize == 48 and INSTANCE.itemsize == 160
 assert PRIMITIVE.itemsize == 100 and CAMERA.itemsize == 128 and GLOBALS.itemsize == 16
-assert REQUEST.itemsize == 160 and COMPLETION.itemsize == 32 and SERVER_STATS.itemsize == 96
+assert REQUEST.itemsize == 176 and COMPLETION.itemsize == 32 and SERVER_STATS.itemsize == 112
 
 
 def mat4_cols(m):

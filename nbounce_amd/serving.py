@@ -22,11 +22,18 @@ class Server:
         self._scenes.append((scene_buffers, envmap))
         return idx.value
 
-    def submit(self, camera_data, scene, width, height, spp, bounces=8, contribution_factor=4.0, arrival_s=-1.0):
+    def set_cache(self, scene, cache):
+        """Cached requests for `scene` use (and train) this RadianceCache."""
+        check(lib.nb_server_set_cache(self.handle, scene, cache.handle))
+        self._scenes.append(cache)
+
+    def submit(self, camera_data, scene, width, height, spp, bounces=8, contribution_factor=4.0, arrival_s=-1.0,
+               cached=False, train_paths=0, lr=1e-3):
         r = np.zeros(1, dtype=layout.REQUEST)
         r["camera"] = camera_data
         r["scene"], r["width"], r["height"], r["spp"] = scene, width, height, spp
         r["bounces"], r["contribution_factor"], r["arrival_s"] = bounces, contribution_factor, arrival_s
+        r["mode"], r["train_paths"], r["lr"] = (1 if cached else 0), train_paths, lr
         rid = ctypes.c_uint64()
         check(lib.nb_server_submit(self.handle, ptr(r), ctypes.byref(rid)))
         return rid.value
@@ -96,7 +103,8 @@ def mixed_requests(n, qps, n_scenes, seed=0, extent=200.0):
     return reqs
 
 
-def run_open_loop(server, requests, camera_data_fn, bounces=8, contribution_factor=4.0, keep_images=False):
+def run_open_loop(server, requests, camera_data_fn, bounces=8, contribution_factor=4.0, keep_images=False,
+                  train_paths=0, lr=1e-3):
     """Drives the server in real time: submits each request when its arrival time has passed,
     steps while there is work, and collects completions.  Returns (latencies, wall seconds,
     images by request index if keep_images)."""
@@ -111,7 +119,8 @@ def run_open_loop(server, requests, camera_data_fn, bounces=8, contribution_fact
         while i < n and requests[i]["arrival"] <= now:
             r = requests[i]
             rid = server.submit(camera_data_fn(r), r["scene"], r["width"], r["height"], r["spp"], bounces,
-                                contribution_factor, arrival_s=t0 + r["arrival"])
+                                contribution_factor, arrival_s=t0 + r["arrival"], cached=r.get("cached", False),
+                                train_paths=train_paths, lr=lr)
             ids[rid] = i
             i += 1
         active, queued = server.step()

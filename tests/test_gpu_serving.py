@@ -155,3 +155,53 @@ def test_small_requests_overtake_a_long_one(env):
     assert set(order) == set(small) | {big}
     assert order.index(big) == len(order) - 1
     srv.close()
+
+
+def _cached_reference(nb, ctx, sb, e, cache, cam, w, h, spp, bounces):
+    """nb_render_cached for samples 1..spp, weight 1/sample, on a fresh target."""
+    import ctypes
+    from nbounce_amd._lib import check, lib, ptr
+    t = ctypes.c_void_p()
+    check(lib.nb_target_create(ctx.handle, w, h, ctypes.byref(t)))
+    for k in range(1, spp + 1):
+        g = nb.Globals(k, float(np.float32(1.0) / np.float32(k)), bounces, 4.0).array()
+        check(lib.nb_render_cached(ctx.handle, sb.handle, t, ptr(cam), ptr(g), e.handle, cache.handle, 0, 0.0, None))
+    img = np.zeros((h, w, 4), dtype=np.float32)
+    check(lib.nb_target_read(t, ptr(img)))
+    lib.nb_target_destroy(t)
+    return img
+
+
+@pytest.mark.parametrize("fp16", [False, True])
+def test_cached_request_equals_render_cached_and_plain_ones_are_unchanged(env, fp16):
+    """A cache-assisted request (inference only) equals nb_render_cached of its samples on the
+    scene's cache, and plain requests served in the same steps still equal nb_render_frames."""
+    from nbounce_amd.cache import RadianceCache
+    from nbounce_amd.serving import Server
+    nb, ctx, scenes, e = env
+    print(f"case: cached + plain requests in one server, {'fp16' if fp16 else 'bf16'} cache", flush=True)
+    cache = RadianceCache(ctx, [0, 0, 0], [10, 10, 10], seed=4, fp16=fp16)
+    srv = Server(ctx, max_paths=1 << 16, max_bounces=12)
+    for s in scenes:
+        srv.add_scene(s, e)
+    srv.set_cache(0, cache)
+    w, h = 64, 48
+    cam0 = _cam(nb, (5.0, 5.0, 9.5), (5.0, 4.0, 0.0), w, h)
+    cam1 = _cam(nb, REQ["eye"], REQ["target"], REQ["w"], REQ["h"])
+    rc = srv.submit(cam0, 0, w, h, 3, 8, cached=True)
+    rp = srv.submit(cam1, REQ["scene"], REQ["w"], REQ["h"], REQ["spp"], REQ["bounces"])
+    with pytest.raises(nb.NbError):
+        srv.submit(cam1, 1, 32, 32, 1, 8, cached=True)      # scene 1 has no cache
+    done = _run_to_completion(srv)
+    assert rc in done and rp in done
+    got_c = srv.read(rc, w, h)
+    got_p = srv.read(rp, REQ["w"], REQ["h"])
+    st = srv.stats()
+    assert st["cached_samples"] == 3 * w * h and st["cached_busy_s"] > 0
+    srv.close()
+    want_c = _cached_reference(nb, ctx, scenes[0], e, cache, cam0, w, h, 3, 8)
+    want_p = _plain(nb, ctx, scenes[1], e, cam1, REQ["w"], REQ["h"], REQ["spp"], REQ["bounces"])
+    assert np.isfinite(got_c).all() and got_c[..., :3].max() > 0
+    np.testing.assert_array_equal(got_c.view(np.uint32), want_c.view(np.uint32))
+    np.testing.assert_array_equal(got_p.view(np.uint32), want_p.view(np.uint32))
+    cache.close()
