@@ -241,16 +241,19 @@ __device__ __forceinline__ bool inner_pending(const TraceState &ts, const Stack<
     return st.sp != 0 && !(ts.blas_base != kNoRecord && st.sp == ts.blas_base);
 }
 
-// One inner-node visit or plain pop (requires inner_pending).
+// A plain pop (when the lane has no node) followed by the visit of the popped node if it is an
+// inner node still nearer than the hit, or the visit alone (requires inner_pending).  The same
+// operations in the same order as trace_step; a pop and the visit it leads to share one
+// iteration, so lanes that pop and lanes that descend run the visit code together.
 template <int K>
 __device__ __forceinline__ void inner_step(const DevScene &S, TraceState &ts, Stack<K> &st, LaneCounters &c) {
-    if (ts.have) {
-        visit_inner<K>(S, ts, st, c);
-    } else {
+    if (!ts.have) {
         float t;
         st.pop(ts.cur, t);
         ts.have = t < ts.hit.t;   // skip entries farther than the current hit
+        if (!ts.have || (ts.cur & kLeafBit)) return;
     }
+    visit_inner<K>(S, ts, st, c);
 }
 
 // One traversal step; true when the closest hit of `world` is final in ts.hit.
