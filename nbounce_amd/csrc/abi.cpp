@@ -121,6 +121,8 @@ nb_status relayout(const nb_bvh_node *blas, uint32_t n_blas, const nb_bvh_node *
     NB_TRY(check_tree(tlas, n_tlas, n_inst, "TLAS"));
     if (n_tlas > 0 && n_inst == 0) return set_error(NB_ERR_BAD_LAYOUT, "TLAS without instances");
     if (n_inst > kLeafStartMask) return set_error(NB_ERR_UNSUPPORTED, "more than 2^26 instances");
+    // the kernels address triangles (48 B each) with 32-bit byte offsets
+    if (static_cast<uint64_t>(n_tris) * 48u > 0xFFFFFFFFull) return set_error(NB_ERR_UNSUPPORTED, "more than 89478485 triangles");
     for (uint32_t j = 0; j < n_inst; ++j)
         if (inst[j].node >= n_blas) return set_error(NB_ERR_BAD_LAYOUT, "instance %u: BLAS root %u out of range", j, inst[j].node);
 
@@ -160,6 +162,8 @@ nb_status relayout(const nb_bvh_node *blas, uint32_t n_blas, const nb_bvh_node *
         dfs(tlas, 0, pid_t, 2);
     }
 
+    // the kernels address records with 32-bit byte offsets (64 B each)
+    if (recs.size() > (size_t{1} << 26)) return set_error(NB_ERR_UNSUPPORTED, "more than 2^26 node records");
     R.tri_leaf.assign(n_tris ? n_tris : 1, 0u);
     R.inst_leaf.assign(n_inst ? n_inst : 1, 0u);
     auto pair_of = [&](const nb_bvh_node *nodes, std::vector<uint32_t> &pid, std::vector<uint32_t> &side, uint32_t i,

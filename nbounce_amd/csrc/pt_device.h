@@ -162,6 +162,13 @@ struct Stack {
     }
 };
 
+// Element at byte offset `off` of a device array, in 32-bit arithmetic: node records, triangles
+// and instance records each stay below 4 GiB (checked at upload), and the load takes the array's
+// scalar base plus one 32-bit vector offset.
+__device__ __forceinline__ const float4 *at_byte(const float4 *base, uint32_t off) {
+    return reinterpret_cast<const float4 *>(reinterpret_cast<const char *>(base) + off);
+}
+
 __device__ __forceinline__ uint32_t leaf_count(uint32_t ref, const uint32_t *side) {
     uint32_t c = (ref >> kLeafStartBits) & kLeafMaxInline;
     return c != 0 ? c : side[ref & kLeafStartMask];
@@ -195,30 +202,11 @@ __device__ __forceinline__ void trace_begin(const DevScene &S, const Ray &world,
     ts.have = S.tlas_root != kNoRecord;
 }
 
-// (W2L * (o, 1)).xyz and mat3(W2L) * d of instance ts.inst_cur, raytracing_sw.wgsl:203-205
-__device__ __forceinline__ void enter_instance(const DevScene &S, const Ray &world, TraceState &ts, uint32_t sp,
-                                               LaneCounters &c) {
-    const float4 *rec = S.inst + 4 * static_cast<size_t>(ts.inst_cur);
-    float4 r0 = rec[0], r1 = rec[1], r2 = rec[2];
-    uint32_t root = __float_as_uint(rec[3].x);
-    ts.ray.o = f3(((r0.x * world.o.x + r0.y * world.o.y) + r0.z * world.o.z) + r0.w * 1.0f,
-                  ((r1.x * world.o.x + r1.y * world.o.y) + r1.z * world.o.z) + r1.w * 1.0f,
-                  ((r2.x * world.o.x + r2.y * world.o.y) + r2.z * world.o.z) + r2.w * 1.0f);
-    ts.ray.d = f3((r0.x * world.d.x + r0.y * world.d.y) + r0.z * world.d.z,
-                  (r1.x * world.d.x + r1.y * world.d.y) + r1.z * world.d.z,
-                  (r2.x * world.d.x + r2.y * world.d.y) + r2.z * world.d.z);
-    ts.ray.inv = rcp3(ts.ray.d);
-    ++c.inst;
-    ts.blas_base = sp;
-    ts.cur = root;
-    ts.have = true;
-}
-
 // The inner-node part of a traversal step at record ts.cur: test both children, push the far one
 // if it is nearer than the current hit, continue with the near one (have = true) or not.
 template <int K>
 __device__ __forceinline__ void visit_inner(const DevScene &S, TraceState &ts, Stack<K> &st, LaneCounters &c) {
-    const float4 *rec = S.pairs + kRecordFloat4 * static_cast<size_t>(ts.cur);
+    const float4 *rec = at_byte(S.pairs, ts.cur * (kRecordFloat4 * 16u));
     const float4 q0 = rec[0], q1 = rec[1], q2 = rec[2];
     const uint4 q3 = *reinterpret_cast<const uint4 *>(rec + 3);
     float tl = slab(ts.ray, q0.x, q0.y, q0.z, q0.w, q1.x, q1.y);
@@ -260,20 +248,22 @@ __device__ __forceinline__ void inner_step(const DevScene &S, TraceState &ts, St
 template <int K>
 __device__ __forceinline__ bool trace_step(const DevScene &S, const Ray &world, TraceState &ts, Stack<K> &st,
                                            LaneCounters &c) {
+    // Lanes of a wave take different cases; the ray changes of all of them (instance entry from a
+    // TLAS leaf, entry of a leaf's next instance, return to world space at a BLAS exit) share one
+    // transform and one 1/d at the end, instead of one copy per case.
+    bool enter = false, leave = false, done = false;
     if (ts.have) {
         if (!(ts.cur & kLeafBit)) {
             visit_inner<K>(S, ts, st, c);
-            if (ts.have) return false;
         } else if (ts.blas_base == kNoRecord) {
             // TLAS leaf: instances [start, start + count), each searched in order
             ts.inst_cur = ts.cur & kLeafStartMask;
             ts.inst_end = ts.inst_cur + leaf_count(ts.cur, S.inst_leaf_count);
-            enter_instance(S, world, ts, st.sp, c);
-            return false;
+            enter = true;
         } else {
             const uint32_t start = ts.cur & kLeafStartMask, end = start + leaf_count(ts.cur, S.tri_leaf_count);
             for (uint32_t j = start; j < end; ++j) {
-                const float4 *tr = S.tris + 3 * static_cast<size_t>(j);
+                const float4 *tr = at_byte(S.tris, j * 48u);
                 float4 a = tr[0], b = tr[1], e = tr[2];
                 ++c.tri;
                 // Möller–Trumbore with back-face culling, raytracing_sw.wgsl:51-78
@@ -295,20 +285,51 @@ __device__ __forceinline__ bool trace_step(const DevScene &S, const Ray &world, 
             ts.have = false;
         }
     }
-    // pop the next entry
-    if (ts.blas_base != kNoRecord && st.sp == ts.blas_base) {
-        if (++ts.inst_cur < ts.inst_end) { enter_instance(S, world, ts, st.sp, c); return false; }
-        ts.blas_base = kNoRecord;
-        // back in world space; 1/d is recomputed (the same bits) so callers need not keep it
-        ts.ray.o = world.o;
-        ts.ray.d = world.d;
-        ts.ray.inv = rcp3(world.d);
+    if (!ts.have && !enter) {
+        // pop the next entry; at the bottom of a BLAS go to the leaf's next instance, or back to
+        // world space after its last one
+        if (ts.blas_base != kNoRecord && st.sp == ts.blas_base) {
+            if (++ts.inst_cur < ts.inst_end) {
+                enter = true;
+            } else {
+                ts.blas_base = kNoRecord;
+                leave = true;
+            }
+        }
+        if (!enter) {
+            if (st.sp == 0) {
+                done = true;
+            } else {
+                float t;
+                st.pop(ts.cur, t);
+                ts.have = t < ts.hit.t;   // skip entries farther than the current hit
+            }
+        }
     }
-    if (st.sp == 0) return true;
-    float t;
-    st.pop(ts.cur, t);
-    ts.have = t < ts.hit.t;   // skip entries farther than the current hit
-    return false;
+    if (enter || leave) {
+        // back in world space, 1/d is recomputed (the same bits) so callers need not keep it
+        F3 o = world.o, d = world.d;
+        if (enter) {
+            // (W2L * (o, 1)).xyz and mat3(W2L) * d of instance ts.inst_cur, raytracing_sw.wgsl:203-205
+            const float4 *rec = at_byte(S.inst, ts.inst_cur * 64u);
+            const float4 r0 = rec[0], r1 = rec[1], r2 = rec[2];
+            const uint32_t root = __float_as_uint(rec[3].x);
+            o = f3(((r0.x * world.o.x + r0.y * world.o.y) + r0.z * world.o.z) + r0.w * 1.0f,
+                   ((r1.x * world.o.x + r1.y * world.o.y) + r1.z * world.o.z) + r1.w * 1.0f,
+                   ((r2.x * world.o.x + r2.y * world.o.y) + r2.z * world.o.z) + r2.w * 1.0f);
+            d = f3((r0.x * world.d.x + r0.y * world.d.y) + r0.z * world.d.z,
+                   (r1.x * world.d.x + r1.y * world.d.y) + r1.z * world.d.z,
+                   (r2.x * world.d.x + r2.y * world.d.y) + r2.z * world.d.z);
+            ++c.inst;
+            ts.blas_base = st.sp;
+            ts.cur = root;
+            ts.have = true;
+        }
+        ts.ray.o = o;
+        ts.ray.d = d;
+        ts.ray.inv = rcp3(d);
+    }
+    return done;
 }
 
 // Closest hit of `world` over the two-level BVH (intersect_TLAS / intersect_BLAS,
