@@ -475,23 +475,17 @@ __device__ StepResult shade(const PathConst &pc, uint32_t bounce, const Hit &h, 
                             Vertex &vtx) {
     const DevScene &S = pc.scene;
     const float4 *sh = S.ishade + 6 * static_cast<size_t>(h.inst);
-    float4 s0 = sh[0], s1 = sh[1], s2 = sh[2], s3 = sh[3], s4 = sh[4], s5 = sh[5];
-    F3 albedo = f3(s3.w, s4.w, s5.w);
-    if (s2.w > 0.0f) { out = mul(thr, albedo); return kDone; }
+    const F3 albedo = f3(sh[3].w, sh[4].w, sh[5].w);
+    if (sh[2].w > 0.0f) { out = mul(thr, albedo); return kDone; }
+    const float4 s0 = sh[0], s1 = sh[1], s2 = sh[2];
     const float4 *tr = S.tris + 3 * static_cast<size_t>(h.tri);
-    uint32_t i0 = __float_as_uint(tr[0].w), i1 = __float_as_uint(tr[1].w), i2 = __float_as_uint(tr[2].w);
-    float4 n0 = S.vshade[2 * i0], t0 = S.vshade[2 * i0 + 1];
-    float4 n1 = S.vshade[2 * i1], t1 = S.vshade[2 * i1 + 1];
-    float4 n2 = S.vshade[2 * i2], t2 = S.vshade[2 * i2 + 1];
-    float bx = 1.0f - h.u - h.v, by = h.u, bz = h.v;
+    const uint32_t i0 = __float_as_uint(tr[0].w), i1 = __float_as_uint(tr[1].w), i2 = __float_as_uint(tr[2].w);
+    const float4 n0 = S.vshade[2 * i0], n1 = S.vshade[2 * i1], n2 = S.vshade[2 * i2];
+    const float bx = 1.0f - h.u - h.v, by = h.u, bz = h.v;
     F3 pos = add(ray.o, scale(h.t, ray.d));
     F3 ln = add(add(scale(bx, xyz(n0)), scale(by, xyz(n1))), scale(bz, xyz(n2)));
-    F3 lt = f3((t0.x * bx + t1.x * by) + t2.x * bz, (t0.y * bx + t1.y * by) + t2.y * bz,
-               (t0.z * bx + t1.z * by) + t2.z * bz);
     F3 nrm = f3((s0.x * ln.x + s0.y * ln.y) + s0.z * ln.z, (s1.x * ln.x + s1.y * ln.y) + s1.z * ln.z,
                 (s2.x * ln.x + s2.y * ln.y) + s2.z * ln.z);
-    F3 tan = f3((s3.x * lt.x + s4.x * lt.y) + s5.x * lt.z, (s3.y * lt.x + s4.y * lt.y) + s5.y * lt.z,
-                (s3.z * lt.x + s4.z * lt.y) + s5.z * lt.z);
     float roughness = s0.w, metallic = s1.w;
 
     float alpha = roughness * roughness;
@@ -531,6 +525,14 @@ __device__ StepResult shade(const PathConst &pc, uint32_t bounce, const Hit &h, 
         if (kRecord) vtx.lobe_alpha = alpha;
     } else {
         if (kRecord) vtx.lobe_alpha = 1.0f;
+        // the tangent is read here, where only the diffuse lobe needs it, so that its vertex and
+        // instance data are not held in registers through the rest of the shading
+        const float4 t0 = S.vshade[2 * i0 + 1], t1 = S.vshade[2 * i1 + 1], t2 = S.vshade[2 * i2 + 1];
+        const float4 s3 = sh[3], s4 = sh[4], s5 = sh[5];
+        F3 lt = f3((t0.x * bx + t1.x * by) + t2.x * bz, (t0.y * bx + t1.y * by) + t2.y * bz,
+                   (t0.z * bx + t1.z * by) + t2.z * bz);
+        F3 tan = f3((s3.x * lt.x + s4.x * lt.y) + s5.x * lt.z, (s3.y * lt.x + s4.y * lt.y) + s5.y * lt.z,
+                    (s3.z * lt.x + s4.z * lt.y) + s5.z * lt.z);
         F3 nt = normalize(sub(tan, scale(dot(tan, n), n)));
         F3 b = cross(n, nt);
         float phi = kTwoPi * sob1.y;
