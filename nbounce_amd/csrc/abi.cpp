@@ -584,6 +584,12 @@ nb_status nb_target_create(nb_context *ctx, uint32_t width, uint32_t height, nb_
         return set_error(e == hipErrorOutOfMemory ? NB_ERR_OUT_OF_MEMORY : NB_ERR_HIP, "target allocation: %s",
                          hipGetErrorString(e));
     }
+    if (nb_status st = unit_order_alloc(t); st != NB_OK) {
+        unit_order_free(t);
+        hipFree(t->pixels);
+        delete t;
+        return st;
+    }
     ++ctx->children;
     *out = t;
     return NB_OK;
@@ -594,6 +600,7 @@ nb_status nb_target_destroy(nb_target *t) {
     hipSetDevice(t->ctx->device);
     hipStreamSynchronize(t->ctx->stream);
     hipFree(t->pixels);
+    unit_order_free(t);
     nb_context *ctx = t->ctx;
     delete t;
     child_released(ctx);
@@ -628,7 +635,9 @@ nb_status nb_render(nb_context *ctx, const nb_scene *scene, nb_target *target, c
     NB_TRY(check_render_args(ctx, scene, target, camera, globals, env));
     NB_TRY(use_device(ctx));
     RenderParams p = make_params(ctx, scene, target, camera, globals, env);
-    return timed_frames(ctx, p);
+    NB_TRY(unit_order_prepare(target, scene, camera, globals, p, ctx->stream));
+    NB_TRY(timed_frames(ctx, p));
+    return unit_order_update(target, p, 1u, ctx->stream);
 }
 
 nb_status nb_render_frames(nb_context *ctx, const nb_scene *scene, nb_target *target, const nb_camera *camera,
@@ -641,7 +650,9 @@ nb_status nb_render_frames(nb_context *ctx, const nb_scene *scene, nb_target *ta
     RenderParams p = make_params(ctx, scene, target, camera, first, env);
     p.n_frames = n_frames;
     p.weights_from_sample = 1;
-    return timed_frames(ctx, p);
+    NB_TRY(unit_order_prepare(target, scene, camera, first, p, ctx->stream));
+    NB_TRY(timed_frames(ctx, p));
+    return unit_order_update(target, p, n_frames, ctx->stream);
 }
 
 nb_status nb_last_render_ms(nb_context *ctx, float *ms) {

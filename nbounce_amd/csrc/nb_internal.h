@@ -101,6 +101,8 @@ struct RenderParams {
     uint32_t inner_quorum;   // inner-node steps run alone while more lanes than this have one
     uint32_t tail_units;     // single work units are handed out once a band has at most this many
     uint32_t grab_units;     // work units per grab before that
+    const uint32_t *unit_order;  // position -> work unit, a permutation within each XCD band, or null
+    uint32_t *unit_cost;     // rays traced per work unit, added as pixels finish, or null
 };
 
 // ---- serving (serve.hip, server.cpp) ----------------------------------------------------------
@@ -184,7 +186,37 @@ struct nb_target {
     nb_context *ctx = nullptr;
     float4 *pixels = nullptr;
     uint32_t width = 0, height = 0;
+    // Cost-ordered work units of the persistent kernel (order.hip): rays traced per 16-pixel unit
+    // over the frames since the costs were reset, and the order built from them.
+    uint32_t n_units = 0;
+    uint32_t *unit_cost = nullptr, *unit_order = nullptr, *sort_vals = nullptr;
+    uint8_t *sort_keys = nullptr;   // 2 x n_units
+    int *sort_offsets = nullptr;    // 9 band boundaries
+    void *sort_tmp = nullptr;
+    size_t sort_tmp_bytes = 0;
+    uint32_t cost_frames = 0;       // frames accumulated in unit_cost (halved with the costs)
+    uint32_t view_frames = 0;       // frames rendered since the camera last changed
+    bool order_valid = false;
+    const nb_scene *cost_scene = nullptr;   // the view the costs were gathered with
+    nb_camera cost_camera{};
+    uint32_t cost_bounces = 0;
+    float cost_contribution = 0.0f;
 };
+
+namespace nb {
+// Work-unit order of the persistent kernel (order.hip).  unit_order_alloc / unit_order_free pair
+// with the target's lifetime.  unit_order_prepare, before a launch on `t`, resets the costs when
+// the scene, bounce limit or contribution factor differs from the ones they were gathered with
+// (halves them when only the camera differs), and points p at the order and the cost array (both
+// null with NB_UNIT_ORDER=0).  unit_order_update, after the launch of n frames, rebuilds the order
+// from the costs once the view has held for 2, 4, 8, ... frames.  The order changes only which
+// wave renders a pixel when, never a pixel's samples.
+nb_status unit_order_alloc(nb_target *t);
+void unit_order_free(nb_target *t);
+nb_status unit_order_prepare(nb_target *t, const nb_scene *scene, const nb_camera *cam, const nb_globals *g,
+                             RenderParams &p, hipStream_t s);
+nb_status unit_order_update(nb_target *t, const RenderParams &p, uint32_t n_frames, hipStream_t s);
+}  // namespace nb
 
 struct nb_envmap {
     nb_context *ctx = nullptr;

@@ -145,6 +145,7 @@ __global__ __launch_bounds__(64) void render_pt_kernel(RenderParams p) {
     enum : uint32_t { kIdle = 0, kTracing = 1, kReady = 2 };
     uint32_t mode = kIdle;
     uint32_t x = 0, y = 0, f = 0, bounce = 0, frames_done = 0;
+    uint32_t px_rays = 0;    // rays traced for the lane's pixel in this launch (its unit's cost)
     Ray ray;                 // the path's current ray (world space)
     TraceState ts;
     F3 thr = f3(1.0f, 1.0f, 1.0f), col = f3(0.0f, 0.0f, 0.0f);
@@ -155,6 +156,7 @@ __global__ __launch_bounds__(64) void render_pt_kernel(RenderParams p) {
         if (mode == kReady) {
             const Hit &h = ts.hit;
             ++c.rays;
+            ++px_rays;
             F3 out = f3(0.0f, 0.0f, 0.0f);
             StepResult r;
             if (h.t == kNoHit) {
@@ -179,6 +181,11 @@ __global__ __launch_bounds__(64) void render_pt_kernel(RenderParams p) {
                 if (++f == p.n_frames) {
                     p.target[static_cast<size_t>(y) * p.width + x] = make_float4(col.x, col.y, col.z, 1.0f);
                     mode = kIdle;
+                    if (p.unit_cost) {
+                        const uint32_t tile = (y >> 3) * p.tiles_x + (x >> 3), k = ((y & 7u) << 3) | (x & 7u);
+                        __hip_atomic_fetch_add(p.unit_cost + tile * 4u + (k >> 4), px_rays, __ATOMIC_RELAXED,
+                                               __HIP_MEMORY_SCOPE_AGENT);
+                    }
                 } else {
                     w = 1.0f / static_cast<float>(p.first_sample + f);
                     mode = kTracing;       // next frame of the same pixel: camera ray below
@@ -202,13 +209,17 @@ __global__ __launch_bounds__(64) void render_pt_kernel(RenderParams p) {
                 if (rank < avail) pix = pos + rank;
                 else if (rank - avail < ne - nb) pix = nb + (rank - avail);
                 if (pix != 0xFFFFFFFFu) {
-                    const uint32_t tile = pix >> 6, k = pix & 63u;
+                    // stream position -> work unit (cost order within the band, when given)
+                    const uint32_t unit = p.unit_order ? p.unit_order[pix >> 4] : (pix >> 4);
+                    const uint32_t q = unit * 16u + (pix & 15u);
+                    const uint32_t tile = q >> 6, k = q & 63u;
                     x = (tile % p.tiles_x) * 8u + (k & 7u);
                     y = (tile / p.tiles_x) * 8u + (k >> 3);
                     mode = kTracing;
                     start = true;
                     f = 0;
                     bounce = 0;
+                    px_rays = 0;
                     col = f3(0.0f, 0.0f, 0.0f);
                     // first frame: the caller's weight (nb_render) or 1/sample (nb_render_frames)
                     w = !p.weights_from_sample ? p.first_weight : 1.0f / static_cast<float>(p.first_sample);

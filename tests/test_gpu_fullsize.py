@@ -8,7 +8,8 @@ XCD band, the work stealing at the tail and many waves per CU are in play.
   (two independent schedules of the same samples);
 * 4 frames in one launch equal 4 launches bit for bit;
 * the counted and the counter-free kernel render the same image, and the counts are the same
-  in two passes (a checksum of the work done).
+  in two passes (a checksum of the work done);
+* cost-ordered work units (rebuilt after 1, 2 and 4 frames) and band order render the same image.
 """
 import os
 
@@ -49,20 +50,22 @@ def _tracer(nb, ctx, sb, env, cam, P):
     return pt
 
 
-def _render(pt, frames, mode=None):
-    old = os.environ.get("NB_RENDER")
+def _render(pt, frames, mode=None, env=None):
+    env = dict(env or {})
     if mode:
-        os.environ["NB_RENDER"] = mode
+        env["NB_RENDER"] = mode
+    old = {k: os.environ.get(k) for k in env}
+    os.environ.update(env)
     try:
         for _ in range(frames):
             pt.dispatch()
         return pt.output_texture()
     finally:
-        if mode:
-            if old is None:
-                del os.environ["NB_RENDER"]
+        for k, v in old.items():
+            if v is None:
+                del os.environ[k]
             else:
-                os.environ["NB_RENDER"] = old
+                os.environ[k] = v
 
 
 def test_fullsize_band_matches_oracle(full):
@@ -119,3 +122,12 @@ def test_fullsize_counting_changes_nothing(full):
     assert counts[2]["rays"] == 0
     for im in images[1:]:
         np.testing.assert_array_equal(images[0].view(np.uint32), im.view(np.uint32))
+
+
+def test_fullsize_unit_order_changes_nothing(full):
+    nb, ctx, sb, env, cam, osc, faces, P = full
+    print("case: 1920x1080, 6 frames, cost-ordered work units vs band order", flush=True)
+    a = _render(_tracer(nb, ctx, sb, env, cam, P), 6)
+    b = _render(_tracer(nb, ctx, sb, env, cam, P), 6, env={"NB_UNIT_ORDER": "0"})
+    assert np.isfinite(a).all()
+    np.testing.assert_array_equal(a.view(np.uint32), b.view(np.uint32))

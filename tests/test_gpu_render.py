@@ -203,3 +203,38 @@ def test_pathtracer_surface_caps_and_invalidates(nb, ctx, small_gpu):
     assert not np.array_equal(one, three)
     pt.resize(64, 48)
     assert pt.size == (64, 48)
+
+
+def test_unit_order_across_camera_moves_changes_nothing(nb, ctx, small_gpu):
+    """Cost-ordered work units (costs halved on a camera move, the order kept, rebuilt after two
+    frames of a view) render, frame by frame, what band order renders."""
+    import os
+    w, sb, env = small_gpu
+    from nbounce_amd import scenegen
+    P = scenegen.SMALL
+    eyes = [np.asarray(P["eye"], dtype=np.float64) + np.array([0.3 * k, 0.0, -0.2 * k]) for k in range(3)]
+    print("case: 256x256, 7 frames over 3 camera positions, cost order vs band order", flush=True)
+
+    def run(order):
+        old = os.environ.get("NB_UNIT_ORDER")
+        os.environ["NB_UNIT_ORDER"] = order
+        try:
+            pt, cam = _pathtracer(nb, ctx, sb, env, (256, 256), eyes[0], P["target"])
+            images = []
+            for step, eye_k in enumerate([0, 0, 0, 1, 2, 2, 2]):
+                if step and eye_k != [0, 0, 0, 1, 2, 2, 2][step - 1]:
+                    pt.update(camera=nb.CameraController(world_position=eyes[eye_k], target=P["target"],
+                                                         aspect_ratio=1.0))
+                pt.dispatch()
+                images.append(pt.output_texture())
+            return images
+        finally:
+            if old is None:
+                del os.environ["NB_UNIT_ORDER"]
+            else:
+                os.environ["NB_UNIT_ORDER"] = old
+
+    a, b = run("1"), run("0")
+    for x, y in zip(a, b):
+        assert np.isfinite(x).all()
+        np.testing.assert_array_equal(x.view(np.uint32), y.view(np.uint32))
