@@ -123,11 +123,20 @@ struct ServeRequestDev {
 };
 
 // The samples one request renders in one step: samples first_sample .. first_sample + count - 1
-// of every pixel.  Its paths occupy ids [path_base, path_base + count * pixels), sample-major;
-// its pixels occupy [pixel_base, pixel_base + pixels) of the step's accumulation range.
+// of every pixel.  Its paths (one per sample and pixel) have ids [path_base, path_base + count *
+// pixels), sample-major; its pixels occupy [pixel_base, pixel_base + pixels) of the step's
+// accumulation range.  Its paths form 16-path work units (one sample of two rows of an 8x8 tile,
+// as in the progressive renderer), count * pixels / 16 of them from unit_base in the step's list.
 struct ServePass {
     uint32_t slot, first_sample, count, pixels;
-    uint32_t path_base, pixel_base, pad0, pad1;
+    uint32_t path_base, pixel_base, unit_base, pad;
+};
+
+// The passes of one scene in a step: passes [pass_begin, pass_end), units [unit_begin, unit_end).
+struct ServeGroup {
+    uint32_t scene;
+    uint32_t unit_begin, unit_end, pass_begin, pass_end;
+    uint32_t pad[3];
 };
 
 // Per-step description, at a fixed device address so that one captured graph serves every step.
@@ -135,20 +144,21 @@ struct ServeStep {
     const ServeSceneDev *scenes;
     const ServeRequestDev *requests;
     const ServePass *passes;
-    uint32_t n_passes, n_paths, n_pixels, max_bounces;
+    const ServeGroup *groups;
+    uint32_t n_passes, n_groups, n_units, n_pixels;
     const float4 *sobol;
     uint32_t sobol_entries, sobol_stride;
-    float4 *queue[2];        // path records, 64 B each, ping-pong between bounce levels
     float4 *results;         // radiance per path id
-    uint32_t *level;         // per bounce level: [2*l] = paths queued, [2*l+1] = claimed
+    uint32_t *work;          // per group: units taken (kWorkStride apart), zeroed by the step's reset
     unsigned long long *counters;
+    uint32_t regen_min_idle, inner_quorum, grab_units, pad1;
 };
 
 constexpr uint32_t kServeMaxBounces = 32;
+constexpr uint32_t kServeMaxGroups = 64;   // one per resident scene at most
 
-nb_status serve_launch_reset(const ServeStep *step, uint32_t grid, hipStream_t s);
-nb_status serve_launch_generate(const ServeStep *step, uint32_t grid, hipStream_t s);
-nb_status serve_launch_bounce(const ServeStep *step, uint32_t level, uint32_t grid, hipStream_t s);
+nb_status serve_launch_reset(const ServeStep *step, hipStream_t s);
+nb_status serve_launch_step(const ServeStep *step, hipStream_t s);
 nb_status serve_launch_accumulate(const ServeStep *step, uint32_t grid, hipStream_t s);
 
 // Kernel launchers (trace.hip).

@@ -169,9 +169,23 @@ __device__ __forceinline__ const float4 *at_byte(const float4 *base, uint32_t of
     return reinterpret_cast<const float4 *>(reinterpret_cast<const char *>(base) + off);
 }
 
+// A 16-B load through the global address space.  The scene arrays are global memory; where their
+// pointers come from memory rather than from kernel arguments (the serving kernel reads them
+// from the step description) the compiler cannot tell, and would use flat loads with 64-bit
+// vector addresses.
+typedef float g_v4f __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ float4 ld_global(const float4 *p) {
+    const g_v4f v = *(const __attribute__((address_space(1))) g_v4f *)(p);
+    return make_float4(v.x, v.y, v.z, v.w);
+}
+__device__ __forceinline__ uint4 ld_global_u4(const float4 *p) {
+    const g_v4f v = *(const __attribute__((address_space(1))) g_v4f *)(p);
+    return make_uint4(__float_as_uint(v.x), __float_as_uint(v.y), __float_as_uint(v.z), __float_as_uint(v.w));
+}
+
 __device__ __forceinline__ uint32_t leaf_count(uint32_t ref, const uint32_t *side) {
     uint32_t c = (ref >> kLeafStartBits) & kLeafMaxInline;
-    return c != 0 ? c : side[ref & kLeafStartMask];
+    return c != 0 ? c : *(const __attribute__((address_space(1))) uint32_t *)(side + (ref & kLeafStartMask));
 }
 
 // Two-level BVH traversal in resumable form: one step (an inner node, a TLAS leaf's instance
@@ -207,8 +221,8 @@ __device__ __forceinline__ void trace_begin(const DevScene &S, const Ray &world,
 template <int K>
 __device__ __forceinline__ void visit_inner(const DevScene &S, TraceState &ts, Stack<K> &st, LaneCounters &c) {
     const float4 *rec = at_byte(S.pairs, ts.cur * (kRecordFloat4 * 16u));
-    const float4 q0 = rec[0], q1 = rec[1], q2 = rec[2];
-    const uint4 q3 = *reinterpret_cast<const uint4 *>(rec + 3);
+    const float4 q0 = ld_global(rec), q1 = ld_global(rec + 1), q2 = ld_global(rec + 2);
+    const uint4 q3 = ld_global_u4(rec + 3);
     float tl = slab(ts.ray, q0.x, q0.y, q0.z, q0.w, q1.x, q1.y);
     float tr = slab(ts.ray, q1.z, q1.w, q2.x, q2.y, q2.z, q2.w);
     c.aabb += q3.z ? 1u : 2u;
@@ -222,7 +236,7 @@ __device__ __forceinline__ void visit_inner(const DevScene &S, TraceState &ts, S
 }
 
 // True when the next action of a lane is an inner-node visit or a pop that neither leaves a
-// BLAS nor ends the trace: the cheap, common steps that inner_steps runs.
+// BLAS nor ends the trace: the cheap, common steps that the persistent kernels' inner loop runs.
 template <int K>
 __device__ __forceinline__ bool inner_pending(const TraceState &ts, const Stack<K> &st) {
     if (ts.have) return !(ts.cur & kLeafBit);
@@ -264,7 +278,7 @@ __device__ __forceinline__ bool trace_step(const DevScene &S, const Ray &world, 
             const uint32_t start = ts.cur & kLeafStartMask, end = start + leaf_count(ts.cur, S.tri_leaf_count);
             for (uint32_t j = start; j < end; ++j) {
                 const float4 *tr = at_byte(S.tris, j * 48u);
-                float4 a = tr[0], b = tr[1], e = tr[2];
+                const float4 a = ld_global(tr), b = ld_global(tr + 1), e = ld_global(tr + 2);
                 ++c.tri;
                 // Möller–Trumbore with back-face culling, raytracing_sw.wgsl:51-78
                 F3 e1 = xyz(b), e2 = xyz(e);
@@ -312,8 +326,8 @@ __device__ __forceinline__ bool trace_step(const DevScene &S, const Ray &world, 
         if (enter) {
             // (W2L * (o, 1)).xyz and mat3(W2L) * d of instance ts.inst_cur, raytracing_sw.wgsl:203-205
             const float4 *rec = at_byte(S.inst, ts.inst_cur * 64u);
-            const float4 r0 = rec[0], r1 = rec[1], r2 = rec[2];
-            const uint32_t root = __float_as_uint(rec[3].x);
+            const float4 r0 = ld_global(rec), r1 = ld_global(rec + 1), r2 = ld_global(rec + 2);
+            const uint32_t root = ld_global_u4(rec + 3).x;
             o = f3(((r0.x * world.o.x + r0.y * world.o.y) + r0.z * world.o.z) + r0.w * 1.0f,
                    ((r1.x * world.o.x + r1.y * world.o.y) + r1.z * world.o.z) + r1.w * 1.0f,
                    ((r2.x * world.o.x + r2.y * world.o.y) + r2.z * world.o.z) + r2.w * 1.0f);
@@ -331,6 +345,9 @@ __device__ __forceinline__ bool trace_step(const DevScene &S, const Ray &world, 
     }
     return done;
 }
+
+// Lane states of the persistent kernels.
+enum LaneMode : uint32_t { kIdle = 0, kTracing = 1, kReady = 2 };
 
 // Closest hit of `world` over the two-level BVH (intersect_TLAS / intersect_BLAS,
 // raytracing_sw.wgsl:176-319).
@@ -386,9 +403,10 @@ __device__ __forceinline__ float4 sobol_fetch(const SobolView &t, uint32_t entry
     return make_float4(fract(s.x + shift.x), fract(s.y + shift.y), fract(s.z + shift.z), fract(s.w + shift.w));
 }
 
-// generate_ray, pathtracing.wgsl:27-41 (m = clip_to_world, column-major)
-__device__ __forceinline__ Ray camera_ray(const float *m, uint32_t width, uint32_t height, uint32_t x, uint32_t y,
-                                          float jx, float jy) {
+// generate_ray, pathtracing.wgsl:27-41 (m = clip_to_world, column-major; any pointer to 16 floats)
+template <typename M>
+__device__ __forceinline__ Ray camera_ray(M m, uint32_t width, uint32_t height, uint32_t x, uint32_t y, float jx,
+                                          float jy) {
     float uvx = (2.0f * (static_cast<float>(x) + jx)) / static_cast<float>(width) - 1.0f;
     float uvy = (2.0f * (static_cast<float>(y) + jy)) / static_cast<float>(height) - 1.0f;
     F3 pos = f3(m[12] / m[15], m[13] / m[15], m[14] / m[15]);

@@ -1,13 +1,18 @@
-// Serving kernels: a wavefront form of the integrator for batches of requests that use different
-// scenes, cameras, resolutions and sample counts.  One step runs
-//   reset -> generate -> bounce(0) -> ... -> bounce(max_bounces) -> accumulate
-// as one captured HIP graph; every kernel reads the step from a fixed device address, so the
-// same graph is replayed for every step.  The bounce kernels are persistent: a fixed grid of
-// wave64 workers claims 64 paths at a time from the level's queue, advances each by one bounce
-// (bounce_step, the same arithmetic as the progressive megakernel) and compacts the survivors
-// into the next level's queue with a wave ballot and one atomic per wave.  Each sample is a pure
-// function of (pixel, Sobol row, scene), and the accumulation applies a request's samples in
-// sample order, so a request's image does not depend on what else shares its steps.
+// Serving kernels: one step renders the sample passes the server has planned for every admitted
+// request, for requests that use different scenes, cameras, resolutions and sample counts, as
+// reset -> serve_pt_kernel -> accumulate in one captured HIP graph; the step is described at a
+// fixed device address, so the same graph is replayed for every step.
+//
+// serve_pt_kernel is the progressive renderer's persistent kernel (ray-level regeneration, wave64
+// service and traverse phases; render_pt_kernel in trace.hip) over the step's paths (one per
+// sample and pixel) in 16-path work units, grouped by scene.  A wave serves one scene at a time,
+// so the scene's arrays stay wave-uniform and the traversal is the renderer's; waves start spread
+// over the scenes in proportion to their units, take units from their scene's counter and move
+// on to another scene with units left once their own is exhausted and their lanes have drained.
+// A path's radiance goes to its slot of the results, and the accumulation mixes each pixel's
+// samples into the request's image in sample order with weight 1/sample: every sample is a pure
+// function of (pixel, Sobol row, scene), so a request's image is that of nb_render_frames of its
+// samples alone, whatever shares the step.
 #include "pt_device.h"
 
 namespace nb {
@@ -15,120 +20,191 @@ namespace {
 using namespace dev;
 
 constexpr int kServeK = 16;
+constexpr uint32_t kNoGroup = 0xFFFFFFFFu;
 
-// Path record: q0 = (o.xyz, path id)  q1 = (d.xyz, bounce)  q2 = (thr.xyz, pass index)
-__device__ __forceinline__ uint32_t find_pass(const ServeStep &S, uint32_t path) {
-    uint32_t lo = 0, hi = S.n_passes;
-    while (hi - lo > 1) {
-        uint32_t mid = (lo + hi) >> 1;
-        if (S.passes[mid].path_base <= path) lo = mid; else hi = mid;
-    }
-    return lo;
-}
-
-__device__ __forceinline__ uint32_t find_pass_pixel(const ServeStep &S, uint32_t pix) {
-    uint32_t lo = 0, hi = S.n_passes;
-    while (hi - lo > 1) {
-        uint32_t mid = (lo + hi) >> 1;
-        if (S.passes[mid].pixel_base <= pix) lo = mid; else hi = mid;
-    }
-    return lo;
+// The step's arrays are global memory, but their pointers are read from the step description, so
+// the compiler cannot tell; reads through these pointers are global loads instead of flat ones.
+template <typename T>
+__device__ __forceinline__ const __attribute__((address_space(1))) T *gp(const T *p) {
+    return (const __attribute__((address_space(1))) T *)(p);
 }
 
 __global__ void serve_reset_kernel(const ServeStep *step) {
     const ServeStep &S = *step;
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < 2 * (kServeMaxBounces + 2); i += gridDim.x * blockDim.x)
-        S.level[i] = 0u;
+    for (uint32_t g = threadIdx.x; g < S.n_groups; g += blockDim.x) S.work[g * kWorkStride] = 0u;
 }
 
-__global__ __launch_bounds__(256) void serve_generate_kernel(const ServeStep *step) {
-    const ServeStep &S = *step;
-    if (blockIdx.x == 0 && threadIdx.x == 0) S.level[0] = S.n_paths;
-    const SobolView sob{S.sobol, S.sobol_entries, S.sobol_stride};
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < S.n_paths; i += gridDim.x * blockDim.x) {
-        const uint32_t pi = find_pass(S, i);
-        const ServePass &ps = S.passes[pi];
-        const ServeRequestDev &R = S.requests[ps.slot];
-        const uint32_t local = i - ps.path_base;
-        const uint32_t k = local / ps.pixels, pixel = local % ps.pixels;
-        const uint32_t x = pixel % R.width, y = pixel / R.width;
-        const uint32_t sample = ps.first_sample + k;
-        const float4 shift = hash4f(x, y, x, y);
-        float4 jit = sobol_fetch(sob, sample * S.sobol_stride, shift);
-        Ray r = camera_ray(R.c2w, R.width, R.height, x, y, jit.x, jit.y);
-        float4 *q = S.queue[0] + 4 * static_cast<size_t>(i);
-        q[0] = make_float4(r.o.x, r.o.y, r.o.z, __uint_as_float(i));
-        q[1] = make_float4(r.d.x, r.d.y, r.d.z, __uint_as_float(0u));
-        q[2] = make_float4(1.0f, 1.0f, 1.0f, __uint_as_float(pi));
+// The group holding step unit u (groups cover [0, n_units) in order).
+__device__ __forceinline__ uint32_t group_of_unit(const ServeStep &S, uint32_t u) {
+    uint32_t g = 0;
+    while (g + 1u < S.n_groups && gp(S.groups + g)->unit_end <= u) ++g;
+    return g;
+}
+
+// The pass holding step unit u of group G (its passes are ordered by unit_base).
+__device__ __forceinline__ uint32_t pass_of_unit(const ServeStep &S, uint32_t g, uint32_t u) {
+    uint32_t lo = gp(S.groups + g)->pass_begin, hi = gp(S.groups + g)->pass_end;
+    while (hi - lo > 1u) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (gp(S.passes + mid)->unit_base <= u) lo = mid; else hi = mid;
     }
+    return lo;
 }
 
-__global__ __launch_bounds__(64) void serve_bounce_kernel(const ServeStep *step, uint32_t level) {
-    __shared__ unsigned long long s_stack[kServeK * 64];
+template <int K>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void serve_pt_kernel(const ServeStep *step) {
+    __shared__ unsigned long long s_stack[K * 64];
     const ServeStep &S = *step;
+    if (S.n_units == 0u || S.n_groups == 0u) return;
     const uint32_t lane = threadIdx.x;
-    const uint32_t n_in = S.level[2 * level];
-    const float4 *in = S.queue[level & 1u];
-    float4 *out = S.queue[(level + 1u) & 1u];
     SpillArray spill;
-    Stack<kServeK> st;
+    Stack<K> st;
     st.init(s_stack, lane, spill);
     LaneCounters c{0, 0, 0, 0, 0};
-    uint32_t samples_done = 0;
-    const SobolView sob{S.sobol, S.sobol_entries, S.sobol_stride};
+
+    // the wave's group and scene (wave-uniform)
+    uint32_t g = __builtin_amdgcn_readfirstlane(
+        group_of_unit(S, static_cast<uint32_t>(static_cast<uint64_t>(blockIdx.x) * S.n_units / gridDim.x)));
+    DevScene scene = S.scenes[gp(S.groups + g)->scene].scene;
+    PathConst pc;
+    pc.scene = scene;
+    pc.env = S.scenes[gp(S.groups + g)->scene].env;
+    pc.sobol = SobolView{S.sobol, S.sobol_entries, S.sobol_stride};
+
+    // wave-uniform path stream: step positions [pos, end) (unit * 16 + path of the unit)
+    uint32_t pos = 0, end = 0;
+    bool more = true;   // the wave's group may still hold units
+    uint32_t mode = kIdle;
+    uint32_t x = 0, y = 0, bounce = 0, samples_done = 0;
+    uint32_t path = 0;   // the lane's path id
+    Ray ray;
+    TraceState ts;
+    F3 thr = f3(1.0f, 1.0f, 1.0f);
     for (;;) {
-        uint32_t base = 0;
-        if (lane == 0) base = atomicAdd(&S.level[2 * level + 1], 64u);
-        base = __shfl(base, 0, 64);
-        if (base >= n_in) break;
-        const uint32_t i = base + lane;
-        bool alive = false;
-        float4 q0, q1, q2;
-        if (i < n_in) {
-            const float4 *q = in + 4 * static_cast<size_t>(i);
-            q0 = q[0]; q1 = q[1]; q2 = q[2];
-            const uint32_t path = __float_as_uint(q0.w), bounce = __float_as_uint(q1.w), pi = __float_as_uint(q2.w);
-            const ServePass &ps = S.passes[pi];
-            const ServeRequestDev &R = S.requests[ps.slot];
-            const ServeSceneDev &SC = S.scenes[R.scene];
-            const uint32_t local = path - ps.path_base;
-            const uint32_t pixel = local % ps.pixels;
-            const uint32_t x = pixel % R.width, y = pixel / R.width;
-            PathConst pc;
-            pc.scene = SC.scene;
-            pc.env = SC.env;
-            pc.sobol = sob;
-            pc.sample = ps.first_sample + local / ps.pixels;
-            pc.bounces = R.bounces;
-            pc.contribution_factor = R.contribution_factor;
-            pc.shift = hash4f(x, y, x, y);
-            Ray ray;
-            ray.o = f3(q0.x, q0.y, q0.z);
-            ray.d = f3(q1.x, q1.y, q1.z);
-            ray.inv = rcp3(ray.d);
-            F3 thr = f3(q2.x, q2.y, q2.z), rad = f3(0.0f, 0.0f, 0.0f);
-            StepResult r = bounce_step<kServeK>(pc, bounce, ray, thr, rad, st, c);
-            if (r == kContinue && bounce + 1u > R.bounces) { r = kDone; rad = f3(0.0f, 0.0f, 0.0f); }
-            if (r == kDone) {
-                S.results[path] = make_float4(rad.x, rad.y, rad.z, 1.0f);
-                ++samples_done;
+        bool start = false;
+        // ---- service: shade finished hits
+        if (mode == kReady) {
+            const Hit &h = ts.hit;
+            ++c.rays;
+            F3 out = f3(0.0f, 0.0f, 0.0f);
+            StepResult r;
+            if (h.t == kNoHit) {
+                out = mul(thr, env_sample(pc.env, ray.d));
+                r = kDone;
             } else {
-                alive = true;
-                q0 = make_float4(ray.o.x, ray.o.y, ray.o.z, q0.w);
-                q1 = make_float4(ray.d.x, ray.d.y, ray.d.z, __uint_as_float(bounce + 1u));
-                q2 = make_float4(thr.x, thr.y, thr.z, q2.w);
+                ++c.hits;
+                Vertex dummy;
+                pc.shift = hash4f(x, y, x, y);
+                r = shade<false>(pc, bounce, h, ray, thr, out, dummy);
+                if (r == kContinue && bounce == pc.bounces) { r = kDone; out = f3(0.0f, 0.0f, 0.0f); }
+            }
+            if (r == kContinue) {
+                ++bounce;
+                mode = kTracing;
+                start = true;
+            } else {
+                *(__attribute__((address_space(1))) g_v4f *)(S.results + path) = g_v4f{out.x, out.y, out.z, 1.0f};
+                ++samples_done;
+                mode = kIdle;
             }
         }
-        // wave64 ballot compaction of the surviving paths into the next level's queue
-        const unsigned long long mask = __ballot(alive);
-        const uint32_t n_alive = static_cast<uint32_t>(__popcll(mask));
-        uint32_t obase = 0;
-        if (lane == 0 && n_alive) obase = atomicAdd(&S.level[2 * (level + 1u)], n_alive);
-        obase = __shfl(obase, 0, 64);
-        if (alive) {
-            const uint32_t rank = static_cast<uint32_t>(__popcll(mask & ((1ull << lane) - 1ull)));
-            float4 *q = out + 4 * static_cast<size_t>(obase + rank);
-            q[0] = q0; q[1] = q1; q[2] = q2;
+        // ---- service: idle lanes take the next paths of the group's units, in lane order
+        const unsigned long long idle = __ballot(mode == kIdle);
+        if (idle != 0ull && more) {
+            const uint32_t n_idle = static_cast<uint32_t>(__popcll(idle));
+            const uint32_t avail = end - pos;
+            uint32_t nb = 0, ne = 0;   // units taken now, for the lanes the current ones do not cover
+            if (avail < n_idle) {
+                uint32_t b0 = 0, b1 = 0;
+                if (lane == 0) {
+                    const auto *G = gp(S.groups + g);
+                    const uint32_t units = G->unit_end - G->unit_begin;
+                    const uint32_t i = atomicAdd(S.work + g * kWorkStride, S.grab_units);
+                    if (i < units) {
+                        b0 = (G->unit_begin + i) * 16u;
+                        b1 = (G->unit_begin + min(i + S.grab_units, units)) * 16u;
+                    }
+                }
+                nb = __shfl(b0, 0, 64);
+                ne = __shfl(b1, 0, 64);
+                if (nb == ne) more = false;
+            }
+            if (mode == kIdle) {
+                const uint32_t rank = static_cast<uint32_t>(__popcll(idle & ((1ull << lane) - 1ull)));
+                uint32_t q = 0xFFFFFFFFu;
+                if (rank < avail) q = pos + rank;
+                else if (rank - avail < ne - nb) q = nb + (rank - avail);
+                if (q != 0xFFFFFFFFu) {
+                    // unit -> (pass, sample k of the pass, 16 pixels of the request's image)
+                    const uint32_t unit = q >> 4;
+                    const uint32_t pi = pass_of_unit(S, g, unit);
+                    const auto *P = gp(S.passes + pi);
+                    const auto *R = gp(S.requests + P->slot);
+                    const uint32_t width = R->width, pixels = P->pixels;
+                    const uint32_t units_per_sample = pixels / 16u, local = unit - P->unit_base;
+                    const uint32_t k = local / units_per_sample;
+                    const uint32_t pq = (local - k * units_per_sample) * 16u + (q & 15u);
+                    const uint32_t tiles_x = width / 8u, tile = pq >> 6, kk = pq & 63u;
+                    x = (tile % tiles_x) * 8u + (kk & 7u);
+                    y = (tile / tiles_x) * 8u + (kk >> 3);
+                    path = P->path_base + k * pixels + y * width + x;
+                    pc.sample = P->first_sample + k;
+                    pc.bounces = R->bounces;
+                    pc.contribution_factor = R->contribution_factor;
+                    const float4 jit = sobol_fetch(pc.sobol, pc.sample * S.sobol_stride, hash4f(x, y, x, y));
+                    ray = camera_ray(R->c2w, width, R->height, x, y, jit.x, jit.y);
+                    thr = f3(1.0f, 1.0f, 1.0f);
+                    bounce = 0;
+                    mode = kTracing;
+                    start = true;
+                }
+            }
+            if (n_idle <= avail) {
+                pos += n_idle;
+            } else if (ne > nb) {
+                pos = nb + min(n_idle - avail, ne - nb);
+                end = ne;
+            } else {
+                pos = end;
+            }
+        }
+        // ---- start the traversal of new rays
+        if (start) trace_begin<K>(scene, ray, ts, st);
+        if (__ballot(mode != kIdle) == 0ull) {
+            // every lane idle and the group exhausted: move on to a group with units left, if any
+            uint32_t next = kNoGroup;
+            for (uint32_t k = 1; k <= S.n_groups && next == kNoGroup; ++k) {
+                const uint32_t h = (g + k) % S.n_groups;
+                const auto *G = gp(S.groups + h);
+                const uint32_t taken = __hip_atomic_load(S.work + h * kWorkStride, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                if (taken < G->unit_end - G->unit_begin) next = h;
+            }
+            next = __builtin_amdgcn_readfirstlane(next);
+            if (next == kNoGroup) break;
+            g = next;
+            scene = S.scenes[gp(S.groups + g)->scene].scene;
+            pc.scene = scene;
+            pc.env = S.scenes[gp(S.groups + g)->scene].env;
+            more = true;
+            pos = end = 0;
+            continue;
+        }
+        // ---- traverse
+        const bool more_pixels = more;
+        for (;;) {
+            // inner-node visits and plain pops of every lane that has one, until no more than
+            // inner_quorum lanes do: lanes that reached a leaf, a BLAS exit or the end wait, so
+            // the leaf and instance code below runs for many lanes at once
+            for (;;) {
+                const bool inner = mode == kTracing && inner_pending<K>(ts, st);
+                if (static_cast<uint32_t>(__popcll(__ballot(inner))) <= S.inner_quorum) break;
+                if (inner) inner_step<K>(scene, ts, st, c);
+            }
+            if (mode == kTracing && trace_step<K>(scene, ray, ts, st, c)) mode = kReady;
+            const uint32_t n_trace = static_cast<uint32_t>(__popcll(__ballot(mode == kTracing)));
+            const uint32_t n_ready = static_cast<uint32_t>(__popcll(__ballot(mode == kReady)));
+            const uint32_t n_wait = n_ready + (more_pixels ? 64u - n_trace - n_ready : 0u);
+            if (n_trace == 0 || n_wait >= S.regen_min_idle || (n_trace < S.regen_min_idle && n_wait > 0)) break;
         }
     }
     flush_counters(S.counters, c, samples_done);
@@ -138,16 +214,21 @@ __global__ __launch_bounds__(64) void serve_bounce_kernel(const ServeStep *step,
 __global__ __launch_bounds__(256) void serve_accumulate_kernel(const ServeStep *step) {
     const ServeStep &S = *step;
     for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < S.n_pixels; j += gridDim.x * blockDim.x) {
-        const ServePass &ps = S.passes[find_pass_pixel(S, j)];
+        uint32_t lo = 0, hi = S.n_passes;   // the pass holding accumulation pixel j
+        while (hi - lo > 1u) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (S.passes[mid].pixel_base <= j) lo = mid; else hi = mid;
+        }
+        const ServePass &ps = S.passes[lo];
         const ServeRequestDev &R = S.requests[ps.slot];
         const uint32_t pixel = j - ps.pixel_base;
         float4 *px = R.target + pixel;
-        float4 prev = *px;
+        const float4 prev = *px;
         F3 col = f3(prev.x, prev.y, prev.z);
         for (uint32_t k = 0; k < ps.count; ++k) {
-            const uint32_t sample = ps.first_sample + k;
-            const float w = 1.0f / static_cast<float>(sample);
+            const float w = 1.0f / static_cast<float>(ps.first_sample + k);
             const float4 s = S.results[ps.path_base + k * ps.pixels + pixel];
+            // mix(color, s, weight) = color * (1 - weight) + s * weight
             col = f3(col.x * (1.0f - w) + s.x * w, col.y * (1.0f - w) + s.y * w, col.z * (1.0f - w) + s.z * w);
         }
         *px = make_float4(col.x, col.y, col.z, 1.0f);
@@ -160,26 +241,37 @@ nb_status check_launch(const char *what) {
     return NB_OK;
 }
 
+// Resident waves of serve_pt_kernel on the device (occupancy x CUs), a multiple of 8.
+uint32_t serve_grid() {
+    static const uint32_t grid = [] {
+        int dev = 0, cus = 0, per_cu = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+            hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, serve_pt_kernel<kServeK>, 64, 0) != hipSuccess)
+            return 0u;
+        const uint32_t g = static_cast<uint32_t>(cus) * static_cast<uint32_t>(per_cu < 1 ? 1 : per_cu) / 8u * 8u;
+        return g < 8u ? 8u : g;
+    }();
+    return grid;
+}
+
 }  // namespace
 
-nb_status serve_launch_reset(const ServeStep *step, uint32_t, hipStream_t s) {
-    hipLaunchKernelGGL(serve_reset_kernel, dim3(1), dim3(128), 0, s, step);
+nb_status serve_launch_reset(const ServeStep *step, hipStream_t s) {
+    hipLaunchKernelGGL(serve_reset_kernel, dim3(1), dim3(64), 0, s, step);
     return check_launch("serve reset");
-}
-
-nb_status serve_launch_generate(const ServeStep *step, uint32_t grid, hipStream_t s) {
-    hipLaunchKernelGGL(serve_generate_kernel, dim3(grid), dim3(256), 0, s, step);
-    return check_launch("serve generate");
-}
-
-nb_status serve_launch_bounce(const ServeStep *step, uint32_t level, uint32_t grid, hipStream_t s) {
-    hipLaunchKernelGGL(serve_bounce_kernel, dim3(grid), dim3(64), 0, s, step, level);
-    return check_launch("serve bounce");
 }
 
 nb_status serve_launch_accumulate(const ServeStep *step, uint32_t grid, hipStream_t s) {
     hipLaunchKernelGGL(serve_accumulate_kernel, dim3(grid), dim3(256), 0, s, step);
     return check_launch("serve accumulate");
+}
+
+nb_status serve_launch_step(const ServeStep *step, hipStream_t s) {
+    const uint32_t grid = serve_grid();
+    if (grid == 0u) return set_error(NB_ERR_HIP, "serve kernel occupancy query failed");
+    hipLaunchKernelGGL((serve_pt_kernel<kServeK>), dim3(grid), dim3(64), 0, s, step);
+    return check_launch("serve step");
 }
 
 }  // namespace nb

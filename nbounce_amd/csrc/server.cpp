@@ -1,6 +1,7 @@
 // Serving host side (nb_server_* in include/nbounce.h): resident scenes, request admission by
 // slot and HBM budget, continuous batching of every admitted request's samples into steps, and
-// one captured HIP graph that renders a step.
+// one captured HIP graph that renders a step (reset, one persistent kernel and the ordered
+// accumulation, serve.hip).
 //
 // Scheduling: admitted requests are visited round-robin in admission order; each round gives a
 // request one more sample pass (all its pixels) while the step's path budget lasts, so small
@@ -60,17 +61,18 @@ struct nb_server {
     std::chrono::steady_clock::time_point t0;
 
     // device buffers
-    float4 *queue[2] = {nullptr, nullptr};
-    float4 *results = nullptr;
-    uint32_t *level = nullptr;
+    uint32_t *work = nullptr;                   // per scene group of a step, kWorkStride apart
+    float4 *results = nullptr;                  // radiance per path of a step
     unsigned long long *counters = nullptr;
     ServeStep *d_step = nullptr;
     ServePass *d_passes = nullptr;
+    ServeGroup *d_groups = nullptr;
     ServeRequestDev *d_requests = nullptr;
     ServeSceneDev *d_scenes = nullptr;
     // pinned staging
     ServeStep *h_step = nullptr;
     ServePass *h_passes = nullptr;
+    ServeGroup *h_groups = nullptr;
     ServeRequestDev *h_requests = nullptr;
     ServeSceneDev *h_scenes = nullptr;
     uint32_t n_scenes = 0;
@@ -97,12 +99,10 @@ struct nb_server {
 namespace {
 
 nb_status capture_graph(nb_server *s) {
-    const uint32_t grid_stream = s->cu * 8, grid_bounce = s->cu * 16;
     SRV_HIP(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
-    nb_status st = serve_launch_reset(s->d_step, 1, s->stream);
-    if (st == NB_OK) st = serve_launch_generate(s->d_step, grid_stream, s->stream);
-    for (uint32_t l = 0; st == NB_OK && l <= s->max_bounces; ++l) st = serve_launch_bounce(s->d_step, l, grid_bounce, s->stream);
-    if (st == NB_OK) st = serve_launch_accumulate(s->d_step, grid_stream, s->stream);
+    nb_status st = serve_launch_reset(s->d_step, s->stream);
+    if (st == NB_OK) st = serve_launch_step(s->d_step, s->stream);
+    if (st == NB_OK) st = serve_launch_accumulate(s->d_step, s->cu * 8, s->stream);
     hipGraph_t g = nullptr;
     hipError_t e = hipStreamEndCapture(s->stream, &g);
     if (st != NB_OK) { if (g) hipGraphDestroy(g); return st; }
@@ -120,9 +120,10 @@ void free_server(nb_server *s) {
     for (Request *r : s->queued) delete r;
     for (Request *r : s->active) { hipFree(r->target); delete r; }
     for (auto &kv : s->finished) { hipFree(kv.second->target); delete kv.second; }
-    hipFree(s->queue[0]); hipFree(s->queue[1]); hipFree(s->results); hipFree(s->level); hipFree(s->counters);
-    hipFree(s->d_step); hipFree(s->d_passes); hipFree(s->d_requests); hipFree(s->d_scenes);
-    hipHostFree(s->h_step); hipHostFree(s->h_passes); hipHostFree(s->h_requests); hipHostFree(s->h_scenes);
+    hipFree(s->work); hipFree(s->results); hipFree(s->counters);
+    hipFree(s->d_step); hipFree(s->d_passes); hipFree(s->d_groups); hipFree(s->d_requests); hipFree(s->d_scenes);
+    hipHostFree(s->h_step); hipHostFree(s->h_passes); hipHostFree(s->h_groups); hipHostFree(s->h_requests);
+    hipHostFree(s->h_scenes);
     if (s->ev0) hipEventDestroy(s->ev0);
     if (s->ev1) hipEventDestroy(s->ev1);
     if (s->stream) hipStreamDestroy(s->stream);
@@ -177,22 +178,21 @@ nb_status nb_server_create(nb_context *ctx, uint64_t budget, uint32_t max_paths,
     int cu = 0;
     hipDeviceGetAttribute(&cu, hipDeviceAttributeMultiprocessorCount, ctx->device);
     s->cu = cu > 0 ? static_cast<uint32_t>(cu) : 64u;
-    const size_t qbytes = static_cast<size_t>(max_paths) * 64;
     hipError_t e = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipEventCreate(&s->ev0);
     if (e == hipSuccess) e = hipEventCreate(&s->ev1);
-    if (e == hipSuccess) e = hipMalloc(&s->queue[0], qbytes);
-    if (e == hipSuccess) e = hipMalloc(&s->queue[1], qbytes);
+    if (e == hipSuccess) e = hipMalloc(&s->work, kServeMaxGroups * kWorkStride * sizeof(uint32_t));
     if (e == hipSuccess) e = hipMalloc(&s->results, static_cast<size_t>(max_paths) * 16);
-    if (e == hipSuccess) e = hipMalloc(&s->level, 2 * (kServeMaxBounces + 2) * sizeof(uint32_t));
     if (e == hipSuccess) e = hipMalloc(&s->counters, kCounterBytes);
     if (e == hipSuccess) e = hipMemset(s->counters, 0, kCounterBytes);
     if (e == hipSuccess) e = hipMalloc(&s->d_step, sizeof(ServeStep));
     if (e == hipSuccess) e = hipMalloc(&s->d_passes, kMaxSlots * sizeof(ServePass));
+    if (e == hipSuccess) e = hipMalloc(&s->d_groups, kServeMaxGroups * sizeof(ServeGroup));
     if (e == hipSuccess) e = hipMalloc(&s->d_requests, kMaxSlots * sizeof(ServeRequestDev));
     if (e == hipSuccess) e = hipMalloc(&s->d_scenes, kMaxScenes * sizeof(ServeSceneDev));
     if (e == hipSuccess) e = hipHostMalloc(&s->h_step, sizeof(ServeStep), hipHostMallocDefault);
     if (e == hipSuccess) e = hipHostMalloc(&s->h_passes, kMaxSlots * sizeof(ServePass), hipHostMallocDefault);
+    if (e == hipSuccess) e = hipHostMalloc(&s->h_groups, kServeMaxGroups * sizeof(ServeGroup), hipHostMallocDefault);
     if (e == hipSuccess) e = hipHostMalloc(&s->h_requests, kMaxSlots * sizeof(ServeRequestDev), hipHostMallocDefault);
     if (e == hipSuccess) e = hipHostMalloc(&s->h_scenes, kMaxScenes * sizeof(ServeSceneDev), hipHostMallocDefault);
     if (e != hipSuccess) {
@@ -280,21 +280,42 @@ nb_status nb_server_step(nb_server *s, uint32_t *n_active, uint32_t *n_queued) {
             progress = true;
         }
     }
-    uint64_t pixel_sum = 0;
-    paths = 0;
+    // plain passes grouped by scene (groups in order of first appearance, passes in admission
+    // order within a group); 16-path units numbered across the step in that order
+    uint32_t n_groups = 0;
+    std::vector<int> group_of_scene(s->n_scenes, -1);
+    std::vector<std::vector<size_t>> members;
     for (size_t k = 0; k < s->active.size(); ++k) {
         Request *r = s->active[k];
         if (!take[k] || r->req.mode == NB_REQUEST_CACHED) continue;
-        const uint32_t px = r->req.width * r->req.height;
-        ServePass &p = s->h_passes[n_passes++];
-        p.slot = static_cast<uint32_t>(r->slot);
-        p.first_sample = r->next_sample;
-        p.count = take[k];
-        p.pixels = px;
-        p.path_base = static_cast<uint32_t>(paths);
-        p.pixel_base = static_cast<uint32_t>(pixel_sum);
-        paths += static_cast<uint64_t>(take[k]) * px;
-        pixel_sum += px;
+        int &gi = group_of_scene[r->req.scene];
+        if (gi < 0) { gi = static_cast<int>(n_groups++); members.emplace_back(); }
+        members[gi].push_back(k);
+    }
+    uint32_t units = 0, path_sum = 0, pixel_sum = 0;
+    for (uint32_t gi = 0; gi < n_groups; ++gi) {
+        ServeGroup &G = s->h_groups[gi];
+        G = ServeGroup{};
+        G.scene = s->active[members[gi][0]]->req.scene;
+        G.unit_begin = units;
+        G.pass_begin = n_passes;
+        for (size_t k : members[gi]) {
+            Request *r = s->active[k];
+            ServePass &p = s->h_passes[n_passes++];
+            p = ServePass{};
+            p.slot = static_cast<uint32_t>(r->slot);
+            p.first_sample = r->next_sample;
+            p.count = take[k];
+            p.pixels = r->req.width * r->req.height;
+            p.path_base = path_sum;
+            p.pixel_base = pixel_sum;
+            p.unit_base = units;
+            path_sum += p.count * p.pixels;   // within max_paths (the plan above)
+            pixel_sum += p.pixels;
+            units += p.count * (p.pixels / 16u);
+        }
+        G.unit_end = units;
+        G.pass_end = n_passes;
     }
     bool any = false;
     if (n_passes > 0) {
@@ -302,20 +323,23 @@ nb_status nb_server_step(nb_server *s, uint32_t *n_active, uint32_t *n_queued) {
         st.scenes = s->d_scenes;
         st.requests = s->d_requests;
         st.passes = s->d_passes;
+        st.groups = s->d_groups;
         st.n_passes = n_passes;
-        st.n_paths = static_cast<uint32_t>(paths);
-        st.n_pixels = static_cast<uint32_t>(pixel_sum);
-        st.max_bounces = s->max_bounces;
+        st.n_groups = n_groups;
+        st.n_units = units;
+        st.n_pixels = pixel_sum;
+        st.results = s->results;
         st.sobol = s->ctx->sobol;
         st.sobol_entries = s->ctx->sobol_entries;
         st.sobol_stride = s->ctx->sobol_stride;
-        st.queue[0] = s->queue[0];
-        st.queue[1] = s->queue[1];
-        st.results = s->results;
-        st.level = s->level;
+        st.work = s->work;
         st.counters = s->counters;
+        st.regen_min_idle = 16;   // the progressive renderer's defaults
+        st.inner_quorum = 28;
+        st.grab_units = 4;
         SRV_HIP(hipMemcpyAsync(s->d_requests, s->h_requests, kMaxSlots * sizeof(ServeRequestDev), hipMemcpyHostToDevice, s->stream));
         SRV_HIP(hipMemcpyAsync(s->d_passes, s->h_passes, n_passes * sizeof(ServePass), hipMemcpyHostToDevice, s->stream));
+        SRV_HIP(hipMemcpyAsync(s->d_groups, s->h_groups, n_groups * sizeof(ServeGroup), hipMemcpyHostToDevice, s->stream));
         SRV_HIP(hipMemcpyAsync(s->d_step, s->h_step, sizeof(ServeStep), hipMemcpyHostToDevice, s->stream));
         SRV_HIP(hipEventRecord(s->ev0, s->stream));
         SRV_HIP(hipGraphLaunch(s->graph, s->stream));
