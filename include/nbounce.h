@@ -228,8 +228,9 @@ nb_status nb_primary_hits(nb_context *ctx, const nb_scene *scene, const nb_camer
  * No counterpart in the reference (its frame loop, app.rs:177-219, renders one scene for one
  * window).  A server keeps several scenes resident, admits render requests while earlier ones
  * are on the GPU (as request slots and HBM budget free up), and renders every admitted request's
- * samples together in steps: one captured HIP graph per step (path generation, one persistent
- * wavefront kernel per bounce level with wave64 ballot compaction, ordered accumulation).  Each
+ * samples together in steps: one captured HIP graph per step (one persistent kernel with wave64
+ * ray regeneration over every admitted request's paths of the step, then the ordered
+ * accumulation).  Each
  * request's image equals what nb_render_frames produces for it alone (same camera, size,
  * samples 1..spp, bounces, contribution factor, the context's Sobol table).  A server is used
  * from one host thread at a time. */
