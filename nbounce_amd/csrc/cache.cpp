@@ -58,6 +58,9 @@ uint64_t splitmix(uint64_t &s) {
 struct nb_cache {
     nb_context *ctx = nullptr;
     hipStream_t stream = nullptr;
+    hipStream_t side = nullptr;                    // training paths beside a frame's query tracing
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+    uint32_t *h_count = nullptr;                   // pinned: the training records of a frame
     uint32_t max_batch = 0;
     float scene_min[3], scene_inv[3];
     float *params = nullptr, *adam_m = nullptr, *adam_v = nullptr, *grad = nullptr;
@@ -171,7 +174,10 @@ nb_status nb_cache_create_ex(nb_context *ctx, const float scene_min[3], const fl
     for (int d = 0; d < 3; ++d) { c->scene_min[d] = scene_min[d]; c->scene_inv[d] = 1.0f / (scene_max[d] - scene_min[d]); }
     const size_t B = c->max_batch;
     c->stream = ctx->stream;      // the cache's work is ordered with the context's renders
-    hipError_t e = hipSuccess;
+    hipError_t e = hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipHostMalloc(&c->h_count, 16, hipHostMallocDefault);
     auto alloc = [&](auto **p, size_t bytes) { if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void **>(p), bytes); };
     alloc(&c->params, kMlpParams * 4); alloc(&c->adam_m, kMlpParams * 4); alloc(&c->adam_v, kMlpParams * 4);
     alloc(&c->grad, kMlpParams * 4); alloc(&c->wbf, kBf16Weights * 2); alloc(&c->b5, kMlpOutPad * 4);
@@ -214,6 +220,10 @@ nb_status nb_cache_destroy(nb_cache *c) {
     if (!c) return NB_OK;
     hipSetDevice(c->ctx->device);
     hipStreamSynchronize(c->stream);
+    if (c->side) { hipStreamSynchronize(c->side); hipStreamDestroy(c->side); }
+    if (c->ev_fork) hipEventDestroy(c->ev_fork);
+    if (c->ev_join) hipEventDestroy(c->ev_join);
+    if (c->h_count) hipHostFree(c->h_count);
     void *ps[] = {c->params, c->adam_m, c->adam_v, c->grad, c->wbf, c->b5, c->raw, c->target, c->y, c->rgb, c->loss,
                   c->x, c->dy, c->direct, c->q_raw, c->q_thr, c->q_rgb, c->counts, c->scale_state};
     for (void *p : ps) hipFree(p);
@@ -362,10 +372,19 @@ nb_status cache_render_frame(nb_context *ctx, const DevScene &scene, const DevEn
     f.n_tiles = (width / 8) * (height / 8);
     f.tiles_per_xcd = (f.n_tiles + 7) / 8;
     if (train_paths > 0) {
-        C_TRY(cache_launch_train_paths(f, ctx->stream));
-        uint32_t n = 0;
-        C_HIP(hipMemcpyAsync(&n, c->counts + 1, 4, hipMemcpyDeviceToHost, ctx->stream));
-        C_HIP(hipStreamSynchronize(ctx->stream));
+        // The training paths do not depend on the network and neither does the frame's tracing up
+        // to its queries: the two run side by side (the side stream starts after the work already
+        // queued on the context's stream), and the training steps and the inference that uses
+        // their weights follow both.
+        C_HIP(hipEventRecord(c->ev_fork, ctx->stream));
+        C_HIP(hipStreamWaitEvent(c->side, c->ev_fork, 0));
+        C_TRY(cache_launch_train_paths(f, c->side));
+        C_TRY(cache_launch_query(f, ctx->stream));
+        C_HIP(hipMemcpyAsync(c->h_count, c->counts + 1, 4, hipMemcpyDeviceToHost, c->side));
+        C_HIP(hipEventRecord(c->ev_join, c->side));
+        C_HIP(hipStreamWaitEvent(ctx->stream, c->ev_join, 0));
+        C_HIP(hipEventSynchronize(c->ev_join));
+        uint32_t n = *c->h_count;
         if (n > c->max_batch) n = c->max_batch;
         const uint32_t batch = 16384;
         for (uint32_t off = 0; off + 256 <= n; off += batch) {
@@ -374,8 +393,10 @@ nb_status cache_render_frame(nb_context *ctx, const DevScene &scene, const DevEn
             C_TRY(cache_adam_step(c, lr));
         }
         if (loss) C_HIP(hipMemcpyAsync(loss, c->loss, 4, hipMemcpyDeviceToHost, ctx->stream));
+    } else {
+        C_TRY(cache_launch_query(f, ctx->stream));
     }
-    return cache_launch_frame(f, c->dev(), c->q_rgb, ctx->stream);
+    return cache_launch_finish(f, c->dev(), c->q_rgb, ctx->stream);
 }
 
 nb_context *cache_context(const nb_cache *c) { return c->ctx; }

@@ -211,12 +211,14 @@ nb_status ok_or(const char *what) {
 
 }  // namespace
 
-nb_status cache_launch_frame(const CacheFrame &p, const MlpDev &m, float *rgb, hipStream_t s) {
+nb_status cache_launch_query(const CacheFrame &p, hipStream_t s) {
     hipLaunchKernelGGL(zero_u32_kernel, dim3(1), dim3(64), 0, s, p.q_count, 1u);
     hipLaunchKernelGGL(cache_query_kernel, dim3(8u * p.tiles_per_xcd), dim3(64), 0, s, p);
-    nb_status st = ok_or("cache query");
-    if (st != NB_OK) return st;
-    st = mlp_launch_infer_count(m, p.q_raw, p.q_count, p.width * p.height, rgb, s);
+    return ok_or("cache query");
+}
+
+nb_status cache_launch_finish(const CacheFrame &p, const MlpDev &m, float *rgb, hipStream_t s) {
+    nb_status st = mlp_launch_infer_count(m, p.q_raw, p.q_count, p.width * p.height, rgb, s);
     if (st != NB_OK) return st;
     hipLaunchKernelGGL(cache_combine_kernel, dim3(1024), dim3(256), 0, s, p.q_thr, p.q_count, rgb, p.direct);
     st = ok_or("cache combine");

@@ -92,6 +92,32 @@ __device__ __forceinline__ float feature(const float *r, int f, const MlpDev &m)
     return 0.0f;
 }
 
+// Features f0 .. f0 + 3 (f0 a multiple of 4) of raw record r, the same values as feature(): the
+// sin/cos features come in pairs of one argument (frequencies k and k + 1), so one sin_cos serves
+// two features.
+__device__ __forceinline__ void feature4(const float *r, int f0, const MlpDev &m, float out[4]) {
+    if (f0 < 84) {
+        float x;
+        int k;
+        if (f0 < 72) {
+            const int d = f0 / 24;
+            k = (f0 % 24) >> 1;
+            x = (r[d] - m.scene_min[d]) * m.scene_inv_extent[d];
+        } else {
+            const int g = f0 - 72, d = g / 4;
+            k = (g % 4) >> 1;
+            x = r[3 + d];
+        }
+        sin_cos(x * (kPi * static_cast<float>(1u << k)), out[0], out[1]);
+        sin_cos(x * (kPi * static_cast<float>(1u << (k + 1))), out[2], out[3]);
+    } else if (f0 < 92) {
+        const int b = f0 == 84 ? 6 : 10;   // normal and roughness, or albedo and metallic
+        for (int i = 0; i < 4; ++i) out[i] = r[b + i];
+    } else {
+        for (int i = 0; i < 4; ++i) out[i] = 0.0f;
+    }
+}
+
 constexpr int kPad = 4;                 // LDS row padding (bf16) against bank conflicts
 constexpr int kS0 = kMlpIn + kPad;      // 100
 constexpr int kSH = kMlpHidden + kPad;  // 68
@@ -133,8 +159,11 @@ __global__ __launch_bounds__(256) void mlp_infer_kernel(MlpDev m, const float *r
             for (int q = 0; q < kMlpRaw; ++q) r[q] = 0.0f;
         }
         s4 x[6];
-        for (int t = 0; t < 6; ++t)
-            for (int i = 0; i < 4; ++i) x[t][i] = static_cast<short>(to16<kHalf>(feature(r, 16 * t + 4 * kq + i, m)));
+        for (int t = 0; t < 6; ++t) {
+            float v[4];
+            feature4(r, 16 * t + 4 * kq, m, v);
+            for (int i = 0; i < 4; ++i) x[t][i] = static_cast<short>(to16<kHalf>(v[i]));
+        }
         s4 h[4];
         for (int mt = 0; mt < 4; ++mt) {
             f4v acc = {0.0f, 0.0f, 0.0f, 0.0f};
@@ -410,7 +439,7 @@ nb_status mlp_launch_dense_bwd_data(bool half, const uint16_t *dz, uint32_t out_
 nb_status mlp_launch_dense_bwd_weight(bool half, const uint16_t *dz, uint32_t out, const uint16_t *h_prev,
                                       uint32_t k_in, uint32_t n, float *dw, float *db, hipStream_t s) {
     const uint32_t out_p = (out + 15) / 16 * 16;
-    const uint32_t chunk = 1024;
+    const uint32_t chunk = 256;   // rows per job: a batch of a few thousand rows still fills the GPU
     const size_t jobs = static_cast<size_t>(out_p / 16) * (k_in / 16) * ((n + chunk - 1) / chunk);
     NB_MLP_LAUNCH(dense_bwd_weight_kernel, blocks_for(jobs, 4), 256, half, dz, out_p, out, h_prev, k_in, n, chunk, dw, db);
     return launched("dense backward (weights)");

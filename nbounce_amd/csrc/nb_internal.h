@@ -282,7 +282,10 @@ struct CacheFrame {
     uint32_t tiles_x, n_tiles, tiles_per_xcd;
 };
 
-nb_status cache_launch_frame(const CacheFrame &p, const MlpDev &m, float *rgb, hipStream_t s);
+// A cache-assisted frame in two parts: the path tracing up to the queries (independent of the
+// network), then inference, combination and accumulation.
+nb_status cache_launch_query(const CacheFrame &p, hipStream_t s);
+nb_status cache_launch_finish(const CacheFrame &p, const MlpDev &m, float *rgb, hipStream_t s);
 nb_status cache_render_frame(nb_context *ctx, const DevScene &scene, const DevEnv &env, float4 *target, uint32_t width,
                              uint32_t height, const float *c2w, const nb_globals &g, nb_cache *c, uint32_t train_paths,
                              float lr, float *loss);
