@@ -55,8 +55,6 @@ uint32_t order_shift() {
     return v;
 }
 
-bool power_of_two(uint32_t v) { return v != 0u && (v & (v - 1u)) == 0u; }
-
 }  // namespace
 
 #define NB_ORDER_HIP(call)                                                                              \
@@ -145,9 +143,11 @@ nb_status unit_order_update(nb_target *t, const RenderParams &p, uint32_t n_fram
     // Rebuild once the view has held for 2, 4, 8, ... frames (and after its first frame when there
     // is no order yet), then every 64 frames: the ranking settles as costs add up, and a camera
     // that moves every frame does not pay for a sort per frame.
-    bool rebuild = false;
-    for (uint32_t f = before + 1u; f <= t->view_frames && !rebuild; ++f)
-        rebuild = (power_of_two(f) && (f >= 2u || !t->order_valid)) || f % 64u == 0u;
+    const uint64_t lo = before, hi = t->view_frames;   // frames (lo, hi] were added
+    const uint64_t first_pow2 = t->order_valid ? 2u : 1u;
+    uint64_t pow2 = first_pow2;
+    while (pow2 <= lo) pow2 <<= 1;                      // the first power of two after lo
+    const bool rebuild = pow2 <= hi || (lo / 64u + 1u) * 64u <= hi;
     if (!rebuild || t->cost_frames == 0) return NB_OK;
     const uint32_t n = t->n_units;
     hipLaunchKernelGGL(order_keys_kernel, dim3((n + 255u) / 256u), dim3(256), 0, s, t->unit_cost, n, t->cost_frames,
