@@ -102,14 +102,21 @@ def test_request_is_the_same_alone_and_in_a_mix(env):
         sc = int(k % 2)
         eye = (5.0, 5.0, 9.5) if sc == 0 else tuple(rng.uniform(0, 200, 3))
         tgt = (5.0, 4.0, 0.0) if sc == 0 else tuple(rng.uniform(0, 200, 3))
-        others.append(mix.submit(_cam(nb, eye, tgt, w, h), sc, w, h, int(rng.integers(1, 9)), int(rng.integers(1, 13))))
+        c_k = _cam(nb, eye, tgt, w, h)
+        spp, bounces = int(rng.integers(1, 9)), int(rng.integers(1, 13))
+        others.append((mix.submit(c_k, sc, w, h, spp, bounces), c_k, sc, w, h, spp, bounces))
         if k == 3:
             rid2 = mix.submit(cam, REQ["scene"], REQ["w"], REQ["h"], REQ["spp"], REQ["bounces"])
             mix.step()     # the request arrives while earlier ones are being rendered
     done = _run_to_completion(mix)
-    assert rid2 in done and all(o in done for o in others)
+    assert rid2 in done and all(o[0] in done for o in others)
     img_mix = mix.read(rid2, REQ["w"], REQ["h"])
     assert mix.stats()["steps"] > 2
+    # every request of the mix (both scenes share steps, so waves move between scenes) equals the
+    # plain renderer's image of it
+    for oid, c_k, sc, w, h, spp, bounces in others:
+        np.testing.assert_array_equal(mix.read(oid, w, h).view(np.uint32),
+                                      _plain(nb, ctx, scenes[sc], e, c_k, w, h, spp, bounces).view(np.uint32))
     mix.close()
     np.testing.assert_array_equal(img_alone.view(np.uint32), img_mix.view(np.uint32))
 
