@@ -92,27 +92,30 @@ __device__ __forceinline__ float feature(const float *r, int f, const MlpDev &m)
     return 0.0f;
 }
 
-// Features f0 .. f0 + 3 (f0 a multiple of 4) of raw record r, the same values as feature(): the
-// sin/cos features come in pairs of one argument (frequencies k and k + 1), so one sin_cos serves
-// two features.
-__device__ __forceinline__ void feature4(const float *r, int f0, const MlpDev &m, float out[4]) {
+// Features f0 .. f0 + 3 (f0 a multiple of 4) of the raw record at `row` (global memory), the same
+// values as feature(): the sin/cos features come in pairs of one argument (frequencies k and
+// k + 1), so one sin_cos serves two features.  The record's values are read by index from memory
+// (an index into a register array that differs between lanes would compile to select chains).
+__device__ __forceinline__ void feature4(const float *row, int f0, const MlpDev &m, float out[4]) {
     if (f0 < 84) {
         float x;
         int k;
         if (f0 < 72) {
             const int d = f0 / 24;
             k = (f0 % 24) >> 1;
-            x = (r[d] - m.scene_min[d]) * m.scene_inv_extent[d];
+            const float lo = d == 0 ? m.scene_min[0] : (d == 1 ? m.scene_min[1] : m.scene_min[2]);
+            const float inv = d == 0 ? m.scene_inv_extent[0] : (d == 1 ? m.scene_inv_extent[1] : m.scene_inv_extent[2]);
+            x = (row[d] - lo) * inv;
         } else {
             const int g = f0 - 72, d = g / 4;
             k = (g % 4) >> 1;
-            x = r[3 + d];
+            x = row[3 + d];
         }
         sin_cos(x * (kPi * static_cast<float>(1u << k)), out[0], out[1]);
         sin_cos(x * (kPi * static_cast<float>(1u << (k + 1))), out[2], out[3]);
     } else if (f0 < 92) {
         const int b = f0 == 84 ? 6 : 10;   // normal and roughness, or albedo and metallic
-        for (int i = 0; i < 4; ++i) out[i] = r[b + i];
+        for (int i = 0; i < 4; ++i) out[i] = row[b + i];
     } else {
         for (int i = 0; i < 4; ++i) out[i] = 0.0f;
     }
@@ -148,20 +151,12 @@ __global__ __launch_bounds__(256) void mlp_infer_kernel(MlpDev m, const float *r
     const uint32_t waves = gridDim.x * 4u;
     for (uint32_t base = (blockIdx.x * 4u + wave) * 16u; base < n; base += waves * 16u) {
         const uint32_t row = base + col;
-        float r[kMlpRaw];
-        if (row < n) {
-            const float4 *p = reinterpret_cast<const float4 *>(raw + static_cast<size_t>(row) * kMlpRaw);
-            for (int q = 0; q < 4; ++q) {
-                float4 v = p[q];
-                r[4 * q] = v.x; r[4 * q + 1] = v.y; r[4 * q + 2] = v.z; r[4 * q + 3] = v.w;
-            }
-        } else {
-            for (int q = 0; q < kMlpRaw; ++q) r[q] = 0.0f;
-        }
+        // rows past n compute the features of row n - 1; their outputs are not stored
+        const float *rowp = raw + static_cast<size_t>(row < n ? row : n - 1u) * kMlpRaw;
         s4 x[6];
         for (int t = 0; t < 6; ++t) {
             float v[4];
-            feature4(r, 16 * t + 4 * kq, m, v);
+            feature4(rowp, 16 * t + 4 * kq, m, v);
             for (int i = 0; i < 4; ++i) x[t][i] = static_cast<short>(to16<kHalf>(v[i]));
         }
         s4 h[4];
