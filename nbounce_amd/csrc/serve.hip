@@ -51,6 +51,30 @@ __device__ __forceinline__ uint32_t pass_of_unit(const ServeStep &S, uint32_t g,
     return lo;
 }
 
+// A scene's array pointers as wave-uniform (scalar) values.  They are read from the step
+// description through vector loads; only an explicit broadcast lets the traversal's loads take a
+// scalar base plus a 32-bit offset instead of a 64-bit address per lane (124 -> 110 VGPRs).
+template <typename T>
+__device__ __forceinline__ const T *uniform_ptr(const T *p) {
+    const uint64_t v = reinterpret_cast<uint64_t>(p);
+    const uint32_t lo = __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(v));
+    const uint32_t hi = __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(v >> 32));
+    return reinterpret_cast<const T *>((static_cast<uint64_t>(hi) << 32) | lo);
+}
+__device__ __forceinline__ DevScene uniform_scene(const DevScene &d) {
+    DevScene u = d;
+    u.pairs = uniform_ptr(d.pairs);
+    u.tris = uniform_ptr(d.tris);
+    u.inst = uniform_ptr(d.inst);
+    u.ishade = uniform_ptr(d.ishade);
+    u.vshade = uniform_ptr(d.vshade);
+    u.tri_leaf_count = uniform_ptr(d.tri_leaf_count);
+    u.inst_leaf_count = uniform_ptr(d.inst_leaf_count);
+    u.tlas_root = __builtin_amdgcn_readfirstlane(d.tlas_root);
+    u.n_pairs = __builtin_amdgcn_readfirstlane(d.n_pairs);
+    return u;
+}
+
 template <int K>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void serve_pt_kernel(const ServeStep *step) {
     __shared__ unsigned long long s_stack[K * 64];
@@ -65,7 +89,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void se
     // the wave's group and scene (wave-uniform)
     uint32_t g = __builtin_amdgcn_readfirstlane(
         group_of_unit(S, static_cast<uint32_t>(static_cast<uint64_t>(blockIdx.x) * S.n_units / gridDim.x)));
-    DevScene scene = S.scenes[gp(S.groups + g)->scene].scene;
+    DevScene scene = uniform_scene(S.scenes[gp(S.groups + g)->scene].scene);
     PathConst pc;
     pc.scene = scene;
     pc.env = S.scenes[gp(S.groups + g)->scene].env;
@@ -182,7 +206,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void se
             next = __builtin_amdgcn_readfirstlane(next);
             if (next == kNoGroup) break;
             g = next;
-            scene = S.scenes[gp(S.groups + g)->scene].scene;
+            scene = uniform_scene(S.scenes[gp(S.groups + g)->scene].scene);
             pc.scene = scene;
             pc.env = S.scenes[gp(S.groups + g)->scene].env;
             more = true;
