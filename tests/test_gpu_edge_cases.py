@@ -1,0 +1,149 @@
+"""GPU parity on the path's edge cases, through the C ABI against the CPU oracle: zero bounces
+(only the primary ray's emitter or sky term, pathtracing.wgsl:164-175), a zero contribution
+factor (Russian roulette ends every path after its first scatter, pathtracing.wgsl:227-236),
+16 bounces (the fixed 17-entry Sobol row stride reads two rows further, pathtracing.wgsl:4,151-159),
+the smallest 8x8 target and non-square targets (pathtracer.rs:192-202), and a two-triangle scene
+with one emitter (one instance per triangle, single-leaf BLASes, back-face culling)."""
+import numpy as np
+import pytest
+
+import oracle
+from tests import _workloads as W
+
+pytestmark = pytest.mark.gpu
+
+REL_TOL = 1e-3   # per-pixel relative error bound for images at matched spp and Sobol rows
+
+
+@pytest.fixture(scope="module")
+def nb():
+    import nbounce_amd
+    return nbounce_amd
+
+
+@pytest.fixture(scope="module")
+def ctx(nb):
+    c = nb.Context(0)
+    yield c
+    c.close()
+
+
+@pytest.fixture(scope="module")
+def small_gpu(nb, ctx):
+    from nbounce_amd import scenegen
+    w = W.small()
+    sb = nb.SceneBuffers(ctx, w["blas"], w["tlas"], w["instances"], w["vertices"], w["indices"])
+    env = nb.EnvMap(ctx, scenegen.sky_cube(64))
+    return w, sb, env
+
+
+def _frames_vs_oracle(nb, ctx, sb, osc, env, size, eye, target, frames, bounces=8, cf=4.0):
+    """Renders `frames` progressive samples on the GPU and with the oracle; returns both images."""
+    cam = nb.CameraController(world_position=eye, target=target, aspect_ratio=size[0] / size[1])
+    pt = nb.Pathtracer(ctx, sb, cam, env, size[0], size[1], resolution_factor=1.0, table=W.table(), sobol_stride=17)
+    pt.globals.bounces = bounces
+    pt.globals.contribution_factor = cf
+    for _ in range(frames):
+        pt.dispatch()
+    got = pt.output_texture()
+    assert got.shape == (size[1], size[0], 4)
+    ref = np.zeros((size[1], size[0], 4), dtype=np.float32)
+    oenv = oracle.Env(env.faces)
+    for k in range(1, frames + 1):
+        g = oracle.globals_(k, np.float32(1.0) / np.float32(k), bounces, cf)
+        ref, _ = oracle.render(osc, cam.data(), g, W.table(), 17, oenv, size[0], size[1], rgba=ref, threads=8)
+    return got, ref
+
+
+def _check(got, ref):
+    err = W.rel_err(got, ref)
+    same = (got.view(np.uint32) == ref.view(np.uint32)).all(axis=-1).mean()
+    print(f"bit-identical pixels {same:.5f}, max rel err {err.max():.3e}", flush=True)
+    assert np.isfinite(got).all()
+    assert err.max() <= REL_TOL
+    assert (got[..., 3] == 1.0).all()   # alpha is forced to 1 (pathtracing.wgsl:258)
+
+
+def test_zero_bounces_match_oracle(nb, ctx, small_gpu):
+    from nbounce_amd import scenegen
+    w, sb, env = small_gpu
+    P = scenegen.SMALL
+    print("case: small scene 64x64, 0 bounces, 2 frames", flush=True)
+    got, ref = _frames_vs_oracle(nb, ctx, sb, W.oracle_scene(w), env, (64, 64), P["eye"], P["target"], 2, bounces=0)
+    _check(got, ref)
+    # only emitters and the sky contribute: some pixels are black (diffuse or specular hits)
+    assert (got[..., :3] == 0).all(axis=-1).any()
+
+
+def test_zero_contribution_factor_matches_oracle(nb, ctx, small_gpu):
+    from nbounce_amd import scenegen
+    w, sb, env = small_gpu
+    P = scenegen.SMALL
+    print("case: small scene 64x64, contribution factor 0, 2 frames", flush=True)
+    got, ref = _frames_vs_oracle(nb, ctx, sb, W.oracle_scene(w), env, (64, 64), P["eye"], P["target"], 2, cf=0.0)
+    _check(got, ref)
+    ctx.reset_counters()
+    cam = nb.CameraController(world_position=P["eye"], target=P["target"], aspect_ratio=1.0)
+    pt = nb.Pathtracer(ctx, sb, cam, env, 64, 64, resolution_factor=1.0, table=W.table(), sobol_stride=17)
+    pt.globals.contribution_factor = 0.0
+    pt.dispatch()
+    pt.output_texture()
+    # with p_continue = 0 every path ends at its first scatter: one ray per sample
+    assert ctx.counters()["rays"] == 64 * 64
+
+
+def test_sixteen_bounces_match_oracle(nb, ctx, small_gpu):
+    from nbounce_amd import scenegen
+    w, sb, env = small_gpu
+    P = scenegen.SMALL
+    print("case: small scene 48x32, 16 bounces, 3 frames", flush=True)
+    got, ref = _frames_vs_oracle(nb, ctx, sb, W.oracle_scene(w), env, (48, 32), P["eye"], P["target"], 3, bounces=16)
+    _check(got, ref)
+
+
+@pytest.mark.parametrize("size", [(8, 8), (24, 16), (8, 40), (200, 8)])
+def test_small_and_non_square_targets_match_oracle(nb, ctx, small_gpu, size):
+    from nbounce_amd import scenegen
+    w, sb, env = small_gpu
+    P = scenegen.SMALL
+    print(f"case: small scene {size[0]}x{size[1]}, 8 bounces, 2 frames", flush=True)
+    got, ref = _frames_vs_oracle(nb, ctx, sb, W.oracle_scene(w), env, size, P["eye"], P["target"], 2)
+    _check(got, ref)
+
+
+def _two_triangles():
+    """A front-facing emissive triangle in front of a larger diffuse one, both facing +z."""
+    v = np.zeros(6, dtype=oracle.VERTEX)
+    v["position"] = [(-1, -1, 0), (1, -1, 0), (0, 1, 0), (-4, -4, -2), (4, -4, -2), (0, 4, -2)]
+    v["normal"] = (0, 0, 1)
+    v["tangent"] = (1, 0, 0, 1)
+    idx = np.arange(6, dtype=np.uint32)
+    pr = np.zeros(2, dtype=oracle.PRIMITIVE)
+    pr["local_to_world"] = np.eye(4, dtype=np.float32).reshape(-1)
+    pr[0]["color"], pr[0]["emissive"] = (3.0, 2.0, 1.0, 1.0), 1.0
+    pr[1]["color"], pr[1]["roughness"], pr[1]["metallic"] = (0.8, 0.7, 0.6, 1.0), 0.5, 0.0
+    pr["index_start"], pr["index_end"] = (0, 3), (3, 6)
+    return v, idx, pr
+
+
+def test_two_triangle_scene_matches_oracle(nb, ctx):
+    from nbounce_amd import scenegen
+    v, i, pr = _two_triangles()
+    blas, tlas, inst, idx = oracle.build_scene(v, i, pr)
+    # the library's host builder makes the same arrays
+    scene = nb.Scene(v, i.copy(), pr)
+    b2, t2, i2 = nb.build_scene(scene)
+    np.testing.assert_array_equal(b2.view(np.uint8), blas.view(np.uint8))
+    np.testing.assert_array_equal(t2.view(np.uint8), tlas.view(np.uint8))
+    np.testing.assert_array_equal(scene.indices, idx)
+    sb = nb.SceneBuffers(ctx, blas, tlas, inst, v, idx)
+    env = nb.EnvMap(ctx, scenegen.sky_cube(32))
+    osc = oracle.Scene(blas, tlas, inst, v, idx)
+    print("case: two-triangle scene 32x32, 8 bounces, 4 frames (front view)", flush=True)
+    got, ref = _frames_vs_oracle(nb, ctx, sb, osc, env, (32, 32), (0.0, 0.0, 4.0), (0.0, 0.0, 0.0), 4)
+    _check(got, ref)
+    # the emitter's colour is seen directly at the image centre
+    np.testing.assert_allclose(got[16, 16, :3], [3.0, 2.0, 1.0], rtol=1e-6)
+    print("case: two-triangle scene 32x32 from behind (both culled: sky only)", flush=True)
+    got, ref = _frames_vs_oracle(nb, ctx, sb, osc, env, (32, 32), (0.0, 0.0, -6.0), (0.0, 0.0, 0.0), 2)
+    _check(got, ref)
