@@ -22,12 +22,9 @@ using namespace dev;
 typedef short s4 __attribute__((ext_vector_type(4)));
 typedef float f4v __attribute__((ext_vector_type(4)));
 
-// binary32 -> bfloat16, round to nearest even
-__device__ __forceinline__ uint16_t f2bf(float f) {
-    uint32_t u = __float_as_uint(f);
-    u += 0x7FFFu + ((u >> 16) & 1u);
-    return static_cast<uint16_t>(u >> 16);
-}
+// binary32 -> bfloat16, round to nearest even (gfx950's v_cvt_pk_bf16_f32; for finite inputs
+// the same bits as adding 0x7FFF plus the kept lowest bit and truncating)
+__device__ __forceinline__ uint16_t f2bf(float f) { return __builtin_bit_cast(uint16_t, static_cast<__bf16>(f)); }
 __device__ __forceinline__ float bf2f(uint16_t h) { return __uint_as_float(static_cast<uint32_t>(h) << 16); }
 
 typedef _Float16 h4 __attribute__((ext_vector_type(4)));
@@ -92,6 +89,18 @@ __device__ __forceinline__ float feature(const float *r, int f, const MlpDev &m)
     return 0.0f;
 }
 
+// sin_cos with the quadrant applied by selects and sign flips instead of a switch, the same bits:
+// the feature index, and with it the quadrant, differs between the lanes of a wave, and a switch
+// compiles to divergent branches.
+__device__ __forceinline__ void sin_cos_select(float x, float &s, float &c) {
+    int q;
+    const float r = reduce_quadrant(x, q);
+    const float ps = poly_sin(r), pc = poly_cos(r);
+    const uint32_t neg_s = (static_cast<uint32_t>(q) & 2u) << 30, neg_c = (static_cast<uint32_t>(q + 1) & 2u) << 30;
+    s = __uint_as_float(__float_as_uint((q & 1) ? pc : ps) ^ neg_s);
+    c = __uint_as_float(__float_as_uint((q & 1) ? ps : pc) ^ neg_c);
+}
+
 // Features f0 .. f0 + 3 (f0 a multiple of 4) of the raw record at `row` (global memory), the same
 // values as feature(): the sin/cos features come in pairs of one argument (frequencies k and
 // k + 1), so one sin_cos serves two features.  The record's values are read by index from memory
@@ -111,8 +120,8 @@ __device__ __forceinline__ void feature4(const float *row, int f0, const MlpDev 
             k = (g % 4) >> 1;
             x = row[3 + d];
         }
-        sin_cos(x * (kPi * static_cast<float>(1u << k)), out[0], out[1]);
-        sin_cos(x * (kPi * static_cast<float>(1u << (k + 1))), out[2], out[3]);
+        sin_cos_select(x * (kPi * static_cast<float>(1u << k)), out[0], out[1]);
+        sin_cos_select(x * (kPi * static_cast<float>(1u << (k + 1))), out[2], out[3]);
     } else if (f0 < 92) {
         const int b = f0 == 84 ? 6 : 10;   // normal and roughness, or albedo and metallic
         for (int i = 0; i < 4; ++i) out[i] = row[b + i];
@@ -154,9 +163,13 @@ __global__ __launch_bounds__(256) void mlp_infer_kernel(MlpDev m, const float *r
         // rows past n compute the features of row n - 1; their outputs are not stored
         const float *rowp = raw + static_cast<size_t>(row < n ? row : n - 1u) * kMlpRaw;
         s4 x[6];
+        // unrolled, so that x is indexed by constants (a loop index into a register array
+        // compiles to select chains)
+#pragma unroll
         for (int t = 0; t < 6; ++t) {
             float v[4];
             feature4(rowp, 16 * t + 4 * kq, m, v);
+#pragma unroll
             for (int i = 0; i < 4; ++i) x[t][i] = static_cast<short>(to16<kHalf>(v[i]));
         }
         s4 h[4];
@@ -165,6 +178,9 @@ __global__ __launch_bounds__(256) void mlp_infer_kernel(MlpDev m, const float *r
             for (int t = 0; t < 6; ++t) acc = mfma<kHalf>(load4(&sw0[(16 * mt + col) * kS0 + 16 * t + 4 * kq]), x[t], acc);
             for (int i = 0; i < 4; ++i) h[mt][i] = static_cast<short>(to16<kHalf>(fmaxf(acc[i] + sb[16 * mt + 4 * kq + i], 0.0f)));
         }
+        // not unrolled: unrolled, the compiler keeps all four layers' weight fragments in
+        // registers (256 VGPRs, 2 waves per SIMD) instead of reading them from LDS
+#pragma unroll 1
         for (int L = 0; L < 4; ++L) {
             s4 g[4];
             for (int mt = 0; mt < 4; ++mt) {
