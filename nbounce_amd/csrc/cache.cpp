@@ -75,7 +75,7 @@ struct nb_cache {
     float *q_raw = nullptr, *q_thr = nullptr, *q_rgb = nullptr;
     uint32_t *counts = nullptr;    // [0] queries, [1] training records
     // batch buffers
-    float *raw = nullptr, *target = nullptr, *y = nullptr, *rgb = nullptr, *loss = nullptr;
+    float *raw = nullptr, *target = nullptr, *rgb = nullptr, *loss = nullptr;
     uint16_t *x = nullptr, *h[5] = {}, *dz[5] = {}, *dy = nullptr;
 
     MlpDev dev() const {
@@ -106,28 +106,38 @@ nb_status cache_refresh(nb_cache *c) {
 nb_status cache_forward_backward(nb_cache *c, const float *raw, const float *target, uint32_t n) {
     const MlpDev m = c->dev();
     hipStream_t s = c->stream;
-    C_TRY(mlp_launch_encode(m, raw, n, c->x, s));
-    const uint16_t *in = c->x;
-    const bool h = c->half;
-    for (int L = 0; L < 5; ++L) {
-        C_TRY(mlp_launch_dense_fwd(h, in, kL.in[L], m.w[L], m.b[L], kMlpHidden, n, nullptr, c->h[L], 1, s));
-        in = c->h[L];
-    }
-    C_TRY(mlp_launch_dense_fwd(h, c->h[4], kMlpHidden, m.w[5], m.b[5], kMlpOut, n, c->y, nullptr, 0, s));
+    MlpTrainRows tr{};
+    tr.target = target;
+    tr.x = c->x;
+    tr.dy = c->dy;
+    for (int L = 0; L < 5; ++L) { tr.h[L] = c->h[L]; tr.dz[L] = c->dz[L]; }
+    tr.loss = c->loss;
+    tr.loss_scale = c->half ? reinterpret_cast<const float *>(c->scale_state) : nullptr;
     C_TRY(mlp_launch_fill(c->loss, 1, 0.0f, s));
-    const float *scale = c->half ? reinterpret_cast<const float *>(c->scale_state) : nullptr;
-    C_TRY(mlp_launch_loss(h, c->y, target, n, nullptr, c->dy, c->loss, scale, s));
     C_TRY(mlp_launch_fill(c->grad, kMlpParams, 0.0f, s));
-    // last layer
-    C_TRY(mlp_launch_dense_bwd_weight(h, c->dy, kMlpOut, c->h[4], kMlpHidden, n, c->grad + kL.w[5], c->grad + kL.b[5], s));
-    C_TRY(mlp_launch_dense_bwd_data(h, c->dy, kMlpOutPad, m.w[5], kMlpHidden, n, c->h[4], c->dz[4], s));
-    for (int L = 4; L >= 1; --L) {
-        C_TRY(mlp_launch_dense_bwd_weight(h, c->dz[L], kMlpHidden, c->h[L - 1], kMlpHidden, n, c->grad + kL.w[L],
-                                          c->grad + kL.b[L], s));
-        C_TRY(mlp_launch_dense_bwd_data(h, c->dz[L], kMlpHidden, m.w[L], kMlpHidden, n, c->h[L - 1], c->dz[L - 1], s));
+    C_TRY(mlp_launch_train_forward(m, raw, n, tr, s));
+    C_TRY(mlp_launch_backward(m, n, tr, s));
+    // layer L's weight gradient: its output gradient rows (dy for the last layer) against its
+    // input rows (x for the first), in jobs of 256 rows so that a batch of a few thousand rows
+    // still fills the GPU
+    MlpGradJobs j{};
+    j.rows_per_chunk = 256;
+    const uint32_t chunks = (n + j.rows_per_chunk - 1) / j.rows_per_chunk;
+    uint32_t jobs = 0;
+    for (int L = 0; L < kMlpLayers; ++L) {
+        const bool last = L == kMlpLayers - 1;
+        j.dz[L] = last ? c->dy : c->dz[L];
+        j.h_prev[L] = L == 0 ? c->x : c->h[L - 1];
+        j.dw[L] = c->grad + kL.w[L];
+        j.db[L] = c->grad + kL.b[L];
+        j.out_p[L] = last ? kMlpOutPad : kMlpHidden;
+        j.out_valid[L] = kL.out[L];
+        j.k_in[L] = kL.in[L];
+        j.job_begin[L] = jobs;
+        jobs += (j.out_p[L] / 16) * (j.k_in[L] / 16) * chunks;
     }
-    C_TRY(mlp_launch_dense_bwd_weight(h, c->dz[0], kMlpHidden, c->x, kMlpIn, n, c->grad + kL.w[0], c->grad + kL.b[0], s));
-    return NB_OK;
+    j.job_begin[kMlpLayers] = jobs;
+    return mlp_launch_weight_grad(c->half, j, n, s);
 }
 
 nb_status cache_adam_step(nb_cache *c, float lr) {
@@ -135,9 +145,15 @@ nb_status cache_adam_step(nb_cache *c, float lr) {
     ++c->adam_t;
     const float c1 = static_cast<float>(1.0 / (1.0 - std::pow(0.9, c->adam_t)));
     const float c2 = static_cast<float>(1.0 / (1.0 - std::pow(0.99, c->adam_t)));
-    C_TRY(mlp_launch_adam(c->params, c->adam_m, c->adam_v, c->grad, kMlpParams, lr, b1, b2, eps, c1, c2,
-                          c->half ? c->scale_state : nullptr, c->stream));
-    return cache_refresh(c);
+    MlpRefresh r{};
+    for (int L = 0; L < kMlpLayers; ++L) {
+        r.w_off[L] = kL.w[L];
+        r.w_size[L] = kL.in[L] * kL.out[L];
+        r.wb_off[L] = kL.wb[L];
+    }
+    r.b5_off = kL.b[kMlpLayers - 1];
+    return mlp_launch_adam_refresh(c->half, c->params, c->adam_m, c->adam_v, c->grad, kMlpParams, lr, b1, b2, eps, c1, c2,
+                                   c->half ? c->scale_state : nullptr, r, c->wbf, c->b5, c->stream);
 }
 
 MlpDev cache_dev(const nb_cache *c) { return c->dev(); }
@@ -181,7 +197,7 @@ nb_status nb_cache_create_ex(nb_context *ctx, const float scene_min[3], const fl
     auto alloc = [&](auto **p, size_t bytes) { if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void **>(p), bytes); };
     alloc(&c->params, kMlpParams * 4); alloc(&c->adam_m, kMlpParams * 4); alloc(&c->adam_v, kMlpParams * 4);
     alloc(&c->grad, kMlpParams * 4); alloc(&c->wbf, kBf16Weights * 2); alloc(&c->b5, kMlpOutPad * 4);
-    alloc(&c->raw, B * kMlpRaw * 4); alloc(&c->target, B * 3 * 4); alloc(&c->y, B * 3 * 4); alloc(&c->rgb, B * 3 * 4);
+    alloc(&c->raw, B * kMlpRaw * 4); alloc(&c->target, B * 3 * 4); alloc(&c->rgb, B * 3 * 4);
     alloc(&c->loss, 4); alloc(&c->x, B * kMlpIn * 2); alloc(&c->dy, B * kMlpOutPad * 2);
     alloc(&c->counts, 16);
     alloc(&c->scale_state, 16);
@@ -224,7 +240,7 @@ nb_status nb_cache_destroy(nb_cache *c) {
     if (c->ev_fork) hipEventDestroy(c->ev_fork);
     if (c->ev_join) hipEventDestroy(c->ev_join);
     if (c->h_count) hipHostFree(c->h_count);
-    void *ps[] = {c->params, c->adam_m, c->adam_v, c->grad, c->wbf, c->b5, c->raw, c->target, c->y, c->rgb, c->loss,
+    void *ps[] = {c->params, c->adam_m, c->adam_v, c->grad, c->wbf, c->b5, c->raw, c->target, c->rgb, c->loss,
                   c->x, c->dy, c->direct, c->q_raw, c->q_thr, c->q_rgb, c->counts, c->scale_state};
     for (void *p : ps) hipFree(p);
     for (int L = 0; L < 5; ++L) { hipFree(c->h[L]); hipFree(c->dz[L]); }

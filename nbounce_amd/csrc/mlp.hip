@@ -134,11 +134,23 @@ constexpr int kPad = 4;                 // LDS row padding (bf16) against bank c
 constexpr int kS0 = kMlpIn + kPad;      // 100
 constexpr int kSH = kMlpHidden + kPad;  // 68
 
+// 4 consecutive 16-bit values (8-byte aligned) to global memory
+__device__ __forceinline__ void store4(uint16_t *p, s4 v) {
+    *reinterpret_cast<uint2 *>(p) = make_uint2((static_cast<uint32_t>(static_cast<uint16_t>(v[0]))) |
+                                                   (static_cast<uint32_t>(static_cast<uint16_t>(v[1])) << 16),
+                                               (static_cast<uint32_t>(static_cast<uint16_t>(v[2]))) |
+                                                   (static_cast<uint32_t>(static_cast<uint16_t>(v[3])) << 16));
+}
+
 // Fused forward pass: weights staged once per workgroup in LDS (47 KB), each wave64 takes 16
-// rows at a time through all six layers in registers.
-template <bool kHalf>
-__global__ __launch_bounds__(256) void mlp_infer_kernel(MlpDev m, const float *raw, uint32_t n_max, const uint32_t *n_dev,
-                                                        float *rgb) {
+// rows at a time through all six layers in registers.  kTrain (one training step's forward
+// pass) also stores the encoded inputs and the five hidden activations as 16-bit rows for the
+// backward pass, and ends each row with the relative L2 loss: the loss sum (times 1/(3n)) and
+// the output gradient 2 (y - t) / (y^2 + 0.01) / (3n), times the loss scale and rounded to
+// 16 bits, padded to 16 channels with zeros.
+template <bool kHalf, bool kTrain>
+__global__ __launch_bounds__(256) void mlp_forward_kernel(MlpDev m, const float *raw, uint32_t n_max, const uint32_t *n_dev,
+                                                          float *rgb, MlpTrainRows tr) {
     const uint32_t n = n_dev ? min(*n_dev, n_max) : n_max;
     __shared__ uint16_t sw0[kMlpHidden * kS0];
     __shared__ uint16_t swh[4][kMlpHidden * kSH];
@@ -158,6 +170,7 @@ __global__ __launch_bounds__(256) void mlp_infer_kernel(MlpDev m, const float *r
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int col = lane & 15, kq = lane >> 4;
     const uint32_t waves = gridDim.x * 4u;
+    float loss_part = 0.0f;
     for (uint32_t base = (blockIdx.x * 4u + wave) * 16u; base < n; base += waves * 16u) {
         const uint32_t row = base + col;
         // rows past n compute the features of row n - 1; their outputs are not stored
@@ -171,12 +184,18 @@ __global__ __launch_bounds__(256) void mlp_infer_kernel(MlpDev m, const float *r
             feature4(rowp, 16 * t + 4 * kq, m, v);
 #pragma unroll
             for (int i = 0; i < 4; ++i) x[t][i] = static_cast<short>(to16<kHalf>(v[i]));
+            if constexpr (kTrain) {
+                if (row < n) store4(tr.x + static_cast<size_t>(row) * kMlpIn + 16 * t + 4 * kq, x[t]);
+            }
         }
         s4 h[4];
         for (int mt = 0; mt < 4; ++mt) {
             f4v acc = {0.0f, 0.0f, 0.0f, 0.0f};
             for (int t = 0; t < 6; ++t) acc = mfma<kHalf>(load4(&sw0[(16 * mt + col) * kS0 + 16 * t + 4 * kq]), x[t], acc);
             for (int i = 0; i < 4; ++i) h[mt][i] = static_cast<short>(to16<kHalf>(fmaxf(acc[i] + sb[16 * mt + 4 * kq + i], 0.0f)));
+            if constexpr (kTrain) {
+                if (row < n) store4(tr.h[0] + static_cast<size_t>(row) * kMlpHidden + 16 * mt + 4 * kq, h[mt]);
+            }
         }
         // not unrolled: unrolled, the compiler keeps all four layers' weight fragments in
         // registers (256 VGPRs, 2 waves per SIMD) instead of reading them from LDS
@@ -188,14 +207,95 @@ __global__ __launch_bounds__(256) void mlp_infer_kernel(MlpDev m, const float *r
                 for (int t = 0; t < 4; ++t) acc = mfma<kHalf>(load4(&swh[L][(16 * mt + col) * kSH + 16 * t + 4 * kq]), h[t], acc);
                 const float *bl = sb + (L + 1) * kMlpHidden;
                 for (int i = 0; i < 4; ++i) g[mt][i] = static_cast<short>(to16<kHalf>(fmaxf(acc[i] + bl[16 * mt + 4 * kq + i], 0.0f)));
+                if constexpr (kTrain) {
+                    if (row < n) store4(tr.h[L + 1] + static_cast<size_t>(row) * kMlpHidden + 16 * mt + 4 * kq, g[mt]);
+                }
             }
             for (int mt = 0; mt < 4; ++mt) h[mt] = g[mt];
         }
         f4v acc = {0.0f, 0.0f, 0.0f, 0.0f};
         for (int t = 0; t < 4; ++t) acc = mfma<kHalf>(load4(&sw5[col * kSH + 16 * t + 4 * kq]), h[t], acc);
-        if (kq == 0 && row < n) {
-            const float *b5 = sb + 5 * kMlpHidden;
-            for (int c = 0; c < kMlpOut; ++c) rgb[3 * static_cast<size_t>(row) + c] = acc[c] + b5[c];
+        const float *b5 = sb + 5 * kMlpHidden;
+        if constexpr (kTrain) {
+            // (y - t)^2 / (y^2 + 0.01) with y in the denominator held constant, and its gradient
+            // 2 (y - t) / den / (3n) times the loss scale
+            s4 dyv = {0, 0, 0, 0};
+            if (kq == 0 && row < n) {
+                const float scale = 1.0f / (3.0f * static_cast<float>(n));
+                const float ls = tr.loss_scale ? *tr.loss_scale : 1.0f;
+                for (int c = 0; c < kMlpOut; ++c) {
+                    const float yv = acc[c] + b5[c], tv = tr.target[3 * static_cast<size_t>(row) + c];
+                    const float den = yv * yv + 0.01f;
+                    const float d = yv - tv;
+                    loss_part += d * d / den;
+                    dyv[c] = static_cast<short>(to16<kHalf>(2.0f * d / den * scale * ls));
+                }
+            }
+            if (row < n) store4(tr.dy + static_cast<size_t>(row) * kMlpOutPad + 4 * kq, dyv);
+        } else {
+            if (kq == 0 && row < n)
+                for (int c = 0; c < kMlpOut; ++c) rgb[3 * static_cast<size_t>(row) + c] = acc[c] + b5[c];
+        }
+    }
+    if constexpr (kTrain) {
+        for (int o = 32; o > 0; o >>= 1) loss_part += __shfl_xor(loss_part, o, 64);
+        if (lane == 0 && loss_part != 0.0f) atomicAdd(tr.loss, loss_part * (1.0f / (3.0f * static_cast<float>(n))));
+    }
+}
+
+// Backward pass through the data path of one training step: dH_{L-1}^T = W_L^T dZ_L^T, masked by
+// ReLU'(h_{L-1}) and rounded to 16 bits, for L = 5 .. 1, each wave taking 16 rows at a time with
+// the gradients in registers (the same lane layout as the forward pass) and the weights of layers
+// 1-5 staged in LDS.  Stores dz_0 .. dz_4 (n x 64, 16-bit) for the weight gradients, starting
+// from the forward pass's output gradient dy; each dz value is the fp32 sum over k tiles in order,
+// masked and rounded once.
+template <bool kHalf>
+__global__ __launch_bounds__(256) void mlp_backward_kernel(MlpDev m, uint32_t n, MlpTrainRows tr) {
+    __shared__ uint16_t swh[4][kMlpHidden * kSH];
+    __shared__ uint16_t sw5[kMlpOutPad * kSH];
+    for (int L = 0; L < 4; ++L)
+        for (int i = threadIdx.x; i < kMlpHidden * kMlpHidden; i += blockDim.x)
+            swh[L][(i / kMlpHidden) * kSH + i % kMlpHidden] = m.w[L + 1][i];
+    for (int i = threadIdx.x; i < kMlpOutPad * kMlpHidden; i += blockDim.x)
+        sw5[(i / kMlpHidden) * kSH + i % kMlpHidden] = m.w[5][i];
+    __syncthreads();
+
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int col = lane & 15, kq = lane >> 4;
+    const uint32_t waves = gridDim.x * 4u;
+    for (uint32_t base = (blockIdx.x * 4u + wave) * 16u; base < n; base += waves * 16u) {
+        const uint32_t row = base + col;
+        const bool valid = row < n;
+        const size_t hrow = static_cast<size_t>(valid ? row : 0u) * kMlpHidden;
+        const s4 zero = {0, 0, 0, 0};
+        // layer 5: one k tile over the 16 padded outputs
+        const s4 g5 = valid ? load4(tr.dy + static_cast<size_t>(row) * kMlpOutPad + 4 * kq) : zero;
+        s4 dz[4];
+        for (int mt = 0; mt < 4; ++mt) {
+            s4 a;
+            for (int i = 0; i < 4; ++i) a[i] = static_cast<short>(sw5[(4 * kq + i) * kSH + 16 * mt + col]);
+            const f4v acc = mfma<kHalf>(a, g5, f4v{0.0f, 0.0f, 0.0f, 0.0f});
+            const s4 hp = valid ? load4(tr.h[4] + hrow + 16 * mt + 4 * kq) : zero;
+            for (int i = 0; i < 4; ++i)
+                dz[mt][i] = static_cast<short>(to16<kHalf>(from16<kHalf>(static_cast<uint16_t>(hp[i])) > 0.0f ? acc[i] : 0.0f));
+            if (valid) store4(tr.dz[4] + hrow + 16 * mt + 4 * kq, dz[mt]);
+        }
+#pragma unroll 1
+        for (int L = 4; L >= 1; --L) {
+            s4 nz[4];
+            for (int mt = 0; mt < 4; ++mt) {
+                f4v acc = {0.0f, 0.0f, 0.0f, 0.0f};
+                for (int t = 0; t < 4; ++t) {
+                    s4 a;
+                    for (int i = 0; i < 4; ++i) a[i] = static_cast<short>(swh[L - 1][(16 * t + 4 * kq + i) * kSH + 16 * mt + col]);
+                    acc = mfma<kHalf>(a, dz[t], acc);
+                }
+                const s4 hp = valid ? load4(tr.h[L - 1] + hrow + 16 * mt + 4 * kq) : zero;
+                for (int i = 0; i < 4; ++i)
+                    nz[mt][i] = static_cast<short>(to16<kHalf>(from16<kHalf>(static_cast<uint16_t>(hp[i])) > 0.0f ? acc[i] : 0.0f));
+                if (valid) store4(tr.dz[L - 1] + hrow + 16 * mt + 4 * kq, nz[mt]);
+            }
+            for (int mt = 0; mt < 4; ++mt) dz[mt] = nz[mt];
         }
     }
 }
@@ -213,142 +313,66 @@ __global__ void mlp_encode_kernel(MlpDev m, const float *raw, uint32_t n, uint16
     }
 }
 
-// Z = H_in W^T + b for 16-row tiles; W has round_up(out, 16) rows (padding rows zero).
-// Writes z (fp32, n x out) and/or h_out (16-bit, n x out, ReLU applied when relu != 0).
-template <bool kHalf>
-__global__ __launch_bounds__(256) void dense_fwd_kernel(const uint16_t *h_in, uint32_t k_in, const uint16_t *w,
-                                                        const float *b, uint32_t out, uint32_t n, float *z,
-                                                        uint16_t *h_out, int relu) {
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int col = lane & 15, kq = lane >> 4;
-    const uint32_t m_tiles = (out + 15) / 16, k_tiles = k_in / 16;
-    for (uint32_t tile = blockIdx.x * 4u + wave; tile * 16u < n; tile += gridDim.x * 4u) {
-        const uint32_t row = tile * 16u + col;
-        const bool valid = row < n;
-        for (uint32_t mt = 0; mt < m_tiles; ++mt) {
-            f4v acc = {0.0f, 0.0f, 0.0f, 0.0f};
-            for (uint32_t t = 0; t < k_tiles; ++t) {
-                s4 bop = valid ? load4(h_in + static_cast<size_t>(row) * k_in + 16 * t + 4 * kq) : s4{0, 0, 0, 0};
-                acc = mfma<kHalf>(load4(w + static_cast<size_t>(16 * mt + col) * k_in + 16 * t + 4 * kq), bop, acc);
-            }
-            if (!valid) continue;
-            for (int i = 0; i < 4; ++i) {
-                const uint32_t o = 16 * mt + 4 * kq + i;
-                if (o >= out) continue;
-                const float v = acc[i] + b[o];
-                if (z) z[static_cast<size_t>(row) * out + o] = v;
-                if (h_out) h_out[static_cast<size_t>(row) * out + o] = to16<kHalf>(relu ? fmaxf(v, 0.0f) : v);
-            }
-        }
-    }
-}
-
-// Relative L2 loss (y - t)^2 / (y^2 + 0.01) averaged over rows and channels, with y in the
-// denominator held constant; dy = d loss / d y times the loss scale (1 without one), also as
-// 16-bit values padded to 16 channels.
-template <bool kHalf>
-__global__ __launch_bounds__(256) void loss_kernel(const float *y, const float *t, uint32_t n, float *dy,
-                                                   uint16_t *dy_bf, float *loss, const float *loss_scale) {
-    __shared__ float part[256];
-    float acc = 0.0f;
-    const float scale = 1.0f / (3.0f * static_cast<float>(n));
-    const float ls = loss_scale ? *loss_scale : 1.0f;
-    for (uint32_t row = blockIdx.x * blockDim.x + threadIdx.x; row < n; row += gridDim.x * blockDim.x) {
-        for (int c = 0; c < 16; ++c) {
-            float g = 0.0f;
-            if (c < kMlpOut) {
-                const float yv = y[3 * static_cast<size_t>(row) + c], tv = t[3 * static_cast<size_t>(row) + c];
-                const float den = yv * yv + 0.01f;
-                const float d = yv - tv;
-                acc += d * d / den;
-                g = 2.0f * d / den * scale * ls;
-                if (dy) dy[3 * static_cast<size_t>(row) + c] = g;
-            }
-            dy_bf[16 * static_cast<size_t>(row) + c] = to16<kHalf>(g);
-        }
-    }
-    part[threadIdx.x] = acc;
-    __syncthreads();
-    for (int s = 128; s > 0; s >>= 1) {
-        if (threadIdx.x < s) part[threadIdx.x] += part[threadIdx.x + s];
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) atomicAdd(loss, part[0] * scale);
-}
-
-// dH_prev^T = W^T dZ^T, masked by ReLU'(h_prev): dz (n x out_p 16-bit), w (out_p x k_in 16-bit).
-template <bool kHalf>
-__global__ __launch_bounds__(256) void dense_bwd_data_kernel(const uint16_t *dz, uint32_t out_p, const uint16_t *w,
-                                                             uint32_t k_in, uint32_t n, const uint16_t *h_prev,
-                                                             uint16_t *dz_prev) {
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int col = lane & 15, kq = lane >> 4;
-    const uint32_t m_tiles = k_in / 16, k_tiles = out_p / 16;
-    for (uint32_t tile = blockIdx.x * 4u + wave; tile * 16u < n; tile += gridDim.x * 4u) {
-        const uint32_t row = tile * 16u + col;
-        const bool valid = row < n;
-        for (uint32_t mt = 0; mt < m_tiles; ++mt) {
-            f4v acc = {0.0f, 0.0f, 0.0f, 0.0f};
-            for (uint32_t t = 0; t < k_tiles; ++t) {
-                s4 a = load4_strided(w + static_cast<size_t>(16 * t + 4 * kq) * k_in + 16 * mt + col, k_in);
-                s4 bop = valid ? load4(dz + static_cast<size_t>(row) * out_p + 16 * t + 4 * kq) : s4{0, 0, 0, 0};
-                acc = mfma<kHalf>(a, bop, acc);
-            }
-            if (!valid) continue;
-            for (int i = 0; i < 4; ++i) {
-                const size_t idx = static_cast<size_t>(row) * k_in + 16 * mt + 4 * kq + i;
-                dz_prev[idx] = to16<kHalf>(from16<kHalf>(h_prev[idx]) > 0.0f ? acc[i] : 0.0f);
-            }
-        }
-    }
-}
-
 // dW += dZ^T H_prev over rows, db += column sums of dZ: one (16-output x 16-input) tile per wave
-// over a chunk of rows, then fp32 atomics.  Only outputs < out_valid are written.
+// over a chunk of rows, then fp32 atomics.  Only outputs < out_valid are written.  Job `job` of a
+// layer: tile job % tiles over the rows of chunk job / tiles.
 template <bool kHalf>
-__global__ __launch_bounds__(256) void dense_bwd_weight_kernel(const uint16_t *dz, uint32_t out_p, uint32_t out_valid,
-                                                               const uint16_t *h_prev, uint32_t k_in, uint32_t n,
-                                                               uint32_t rows_per_chunk, float *dw, float *db) {
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+__device__ __forceinline__ void weight_grad_job(const uint16_t *dz, uint32_t out_p, uint32_t out_valid, const uint16_t *h_prev,
+                                                uint32_t k_in, uint32_t n, uint32_t rows_per_chunk, float *dw, float *db,
+                                                uint32_t job) {
+    const int lane = threadIdx.x & 63;
     const int col = lane & 15, kq = lane >> 4;
-    const uint32_t m_tiles = out_p / 16, n_tiles = k_in / 16;
-    const uint32_t tiles = m_tiles * n_tiles;
-    const uint32_t chunks = (n + rows_per_chunk - 1) / rows_per_chunk;
-    for (uint32_t job = blockIdx.x * 4u + wave; job < tiles * chunks; job += gridDim.x * 4u) {
-        const uint32_t tile = job % tiles, chunk = job / tiles;
-        const uint32_t mt = tile / n_tiles, nt = tile % n_tiles;
-        const uint32_t r0 = chunk * rows_per_chunk, r1 = min(n, r0 + rows_per_chunk);
-        f4v acc = {0.0f, 0.0f, 0.0f, 0.0f};
-        float bsum = 0.0f;
-        for (uint32_t rb = r0; rb < r1; rb += 16) {
-            s4 a, bop;
-            for (int i = 0; i < 4; ++i) {
-                const uint32_t row = rb + 4 * kq + i;
-                const bool ok = row < r1;
-                a[i] = ok ? static_cast<short>(dz[static_cast<size_t>(row) * out_p + 16 * mt + col]) : short(0);
-                bop[i] = ok ? static_cast<short>(h_prev[static_cast<size_t>(row) * k_in + 16 * nt + col]) : short(0);
-                if (nt == 0 && ok) bsum += from16<kHalf>(static_cast<uint16_t>(a[i]));
-            }
-            acc = mfma<kHalf>(a, bop, acc);
-        }
+    const uint32_t n_tiles = k_in / 16;
+    const uint32_t tiles = (out_p / 16) * n_tiles;
+    const uint32_t tile = job % tiles, chunk = job / tiles;
+    const uint32_t mt = tile / n_tiles, nt = tile % n_tiles;
+    const uint32_t r0 = chunk * rows_per_chunk, r1 = min(n, r0 + rows_per_chunk);
+    f4v acc = {0.0f, 0.0f, 0.0f, 0.0f};
+    float bsum = 0.0f;
+    for (uint32_t rb = r0; rb < r1; rb += 16) {
+        s4 a, bop;
         for (int i = 0; i < 4; ++i) {
-            const uint32_t o = 16 * mt + 4 * kq + i;
-            if (o < out_valid) atomicAdd(&dw[static_cast<size_t>(o) * k_in + 16 * nt + col], acc[i]);
+            const uint32_t row = rb + 4 * kq + i;
+            const bool ok = row < r1;
+            a[i] = ok ? static_cast<short>(dz[static_cast<size_t>(row) * out_p + 16 * mt + col]) : short(0);
+            bop[i] = ok ? static_cast<short>(h_prev[static_cast<size_t>(row) * k_in + 16 * nt + col]) : short(0);
+            if (nt == 0 && ok) bsum += from16<kHalf>(static_cast<uint16_t>(a[i]));
         }
-        if (nt == 0) {
-            // lanes col, col+16, col+32, col+48 hold partial sums of output 16mt+col
-            bsum += __shfl_xor(bsum, 16, 64);
-            bsum += __shfl_xor(bsum, 32, 64);
-            const uint32_t o = 16 * mt + col;
-            if (kq == 0 && o < out_valid) atomicAdd(&db[o], bsum);
-        }
+        acc = mfma<kHalf>(a, bop, acc);
+    }
+    for (int i = 0; i < 4; ++i) {
+        const uint32_t o = 16 * mt + 4 * kq + i;
+        if (o < out_valid) atomicAdd(&dw[static_cast<size_t>(o) * k_in + 16 * nt + col], acc[i]);
+    }
+    if (nt == 0) {
+        // lanes col, col+16, col+32, col+48 hold partial sums of output 16mt+col
+        bsum += __shfl_xor(bsum, 16, 64);
+        bsum += __shfl_xor(bsum, 32, 64);
+        const uint32_t o = 16 * mt + col;
+        if (kq == 0 && o < out_valid) atomicAdd(&db[o], bsum);
     }
 }
 
-// One Adam step on the gradient g / loss scale; skipped when the gradient check found a
-// non-finite value (loss-scaled training only: scale == nullptr means no scaling).
-__global__ void adam_kernel(float *w, float *m, float *v, const float *g, uint32_t n, float lr, float b1, float b2,
-                            float eps, float c1, float c2, const float *scale, const uint32_t *overflow) {
+// The weight and bias gradients of all six layers in one launch: jobs job_begin[L] ..
+// job_begin[L + 1] - 1 are layer L's (a wave's job, and so its layer, is wave-uniform).
+template <bool kHalf>
+__global__ __launch_bounds__(256) void mlp_weight_grad_kernel(MlpGradJobs J, uint32_t n) {
+    const uint32_t wave = threadIdx.x >> 6;
+    for (uint32_t job = blockIdx.x * 4u + wave; job < J.job_begin[kMlpLayers]; job += gridDim.x * 4u) {
+        int L = 0;
+        while (job >= J.job_begin[L + 1]) ++L;
+        weight_grad_job<kHalf>(J.dz[L], J.out_p[L], J.out_valid[L], J.h_prev[L], J.k_in[L], n, J.rows_per_chunk, J.dw[L],
+                               J.db[L], job - J.job_begin[L]);
+    }
+}
+
+// One Adam step on the gradient g / loss scale (skipped when the gradient check found a non-finite
+// value; scale == nullptr means no scaling), then the parameter's 16-bit weight copy (or the padded
+// last-layer bias) from the new value, as to16_kernel writes it.
+template <bool kHalf>
+__global__ void adam_refresh_kernel(float *w, float *m, float *v, const float *g, uint32_t n, float lr, float b1, float b2,
+                                    float eps, float c1, float c2, const float *scale, const uint32_t *overflow,
+                                    MlpRefresh R, uint16_t *wbf, float *b5) {
     if (overflow && *overflow) return;
     const float inv = scale ? 1.0f / *scale : 1.0f;
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
@@ -357,7 +381,11 @@ __global__ void adam_kernel(float *w, float *m, float *v, const float *g, uint32
         const float vi = b2 * v[i] + (1.0f - b2) * gi * gi;
         m[i] = mi;
         v[i] = vi;
-        w[i] -= lr * (mi * c1) / (sqrtf(vi * c2) + eps);
+        const float wi = w[i] - lr * (mi * c1) / (sqrtf(vi * c2) + eps);
+        w[i] = wi;
+        for (int L = 0; L < kMlpLayers; ++L)
+            if (i - R.w_off[L] < R.w_size[L]) wbf[R.wb_off[L] + (i - R.w_off[L])] = to16<kHalf>(wi);
+        if (i - R.b5_off < static_cast<uint32_t>(kMlpOut)) b5[i - R.b5_off] = wi;
     }
 }
 
@@ -411,6 +439,11 @@ uint32_t blocks_for(size_t items, uint32_t per_block, uint32_t cap = 4096) {
         if (half) hipLaunchKernelGGL(kernel<true>, dim3(grid), dim3(block), 0, s, __VA_ARGS__);  \
         else hipLaunchKernelGGL(kernel<false>, dim3(grid), dim3(block), 0, s, __VA_ARGS__);      \
     } while (0)
+#define NB_MLP_LAUNCH2(kernel, flag, grid, block, half, ...)                                                  \
+    do {                                                                                                    \
+        if (half) hipLaunchKernelGGL((kernel<true, flag>), dim3(grid), dim3(block), 0, s, __VA_ARGS__);   \
+        else hipLaunchKernelGGL((kernel<false, flag>), dim3(grid), dim3(block), 0, s, __VA_ARGS__);       \
+    } while (0)
 
 nb_status mlp_launch_encode(const MlpDev &m, const float *raw, uint32_t n, uint16_t *x, hipStream_t s) {
     NB_MLP_LAUNCH(mlp_encode_kernel, blocks_for(static_cast<size_t>(n) * kMlpIn, 256), 256, m.half, m, raw, n, x);
@@ -418,46 +451,37 @@ nb_status mlp_launch_encode(const MlpDev &m, const float *raw, uint32_t n, uint1
 }
 
 nb_status mlp_launch_infer(const MlpDev &m, const float *raw, uint32_t n, float *rgb, hipStream_t s) {
-    NB_MLP_LAUNCH(mlp_infer_kernel, blocks_for(n, 64, 768), 256, m.half, m, raw, n,
-                  static_cast<const uint32_t *>(nullptr), rgb);
+    NB_MLP_LAUNCH2(mlp_forward_kernel, false, blocks_for(n, 64, 768), 256, m.half, m, raw, n,
+                   static_cast<const uint32_t *>(nullptr), rgb, MlpTrainRows{});
     return launched("mlp infer");
 }
 
 nb_status mlp_launch_infer_count(const MlpDev &m, const float *raw, const uint32_t *n_dev, uint32_t n_max, float *rgb,
                                  hipStream_t s) {
-    NB_MLP_LAUNCH(mlp_infer_kernel, blocks_for(n_max, 64, 768), 256, m.half, m, raw, n_max, n_dev, rgb);
+    NB_MLP_LAUNCH2(mlp_forward_kernel, false, blocks_for(n_max, 64, 768), 256, m.half, m, raw, n_max, n_dev, rgb,
+                   MlpTrainRows{});
     return launched("mlp infer");
 }
 
-nb_status mlp_launch_dense_fwd(bool half, const uint16_t *h_in, uint32_t k_in, const uint16_t *w, const float *b,
-                               uint32_t out, uint32_t n, float *z, uint16_t *h_out, int relu, hipStream_t s) {
-    NB_MLP_LAUNCH(dense_fwd_kernel, blocks_for(n, 64), 256, half, h_in, k_in, w, b, out, n, z, h_out, relu);
-    return launched("dense forward");
+nb_status mlp_launch_train_forward(const MlpDev &m, const float *raw, uint32_t n, const MlpTrainRows &tr, hipStream_t s) {
+    NB_MLP_LAUNCH2(mlp_forward_kernel, true, blocks_for(n, 128, 768), 256, m.half, m, raw, n,
+                   static_cast<const uint32_t *>(nullptr), static_cast<float *>(nullptr), tr);
+    return launched("mlp training forward");
 }
 
-nb_status mlp_launch_loss(bool half, const float *y, const float *t, uint32_t n, float *dy, uint16_t *dy_bf,
-                          float *loss, const float *loss_scale, hipStream_t s) {
-    NB_MLP_LAUNCH(loss_kernel, blocks_for(n, 256, 1024), 256, half, y, t, n, dy, dy_bf, loss, loss_scale);
-    return launched("loss");
+nb_status mlp_launch_backward(const MlpDev &m, uint32_t n, const MlpTrainRows &tr, hipStream_t s) {
+    NB_MLP_LAUNCH(mlp_backward_kernel, blocks_for(n, 128, 768), 256, m.half, m, n, tr);
+    return launched("mlp backward");
 }
 
-nb_status mlp_launch_dense_bwd_data(bool half, const uint16_t *dz, uint32_t out_p, const uint16_t *w, uint32_t k_in,
-                                    uint32_t n, const uint16_t *h_prev, uint16_t *dz_prev, hipStream_t s) {
-    NB_MLP_LAUNCH(dense_bwd_data_kernel, blocks_for(n, 64), 256, half, dz, out_p, w, k_in, n, h_prev, dz_prev);
-    return launched("dense backward (data)");
+nb_status mlp_launch_weight_grad(bool half, const MlpGradJobs &j, uint32_t n, hipStream_t s) {
+    NB_MLP_LAUNCH(mlp_weight_grad_kernel, blocks_for(j.job_begin[kMlpLayers], 4), 256, half, j, n);
+    return launched("mlp weight gradients");
 }
 
-nb_status mlp_launch_dense_bwd_weight(bool half, const uint16_t *dz, uint32_t out, const uint16_t *h_prev,
-                                      uint32_t k_in, uint32_t n, float *dw, float *db, hipStream_t s) {
-    const uint32_t out_p = (out + 15) / 16 * 16;
-    const uint32_t chunk = 256;   // rows per job: a batch of a few thousand rows still fills the GPU
-    const size_t jobs = static_cast<size_t>(out_p / 16) * (k_in / 16) * ((n + chunk - 1) / chunk);
-    NB_MLP_LAUNCH(dense_bwd_weight_kernel, blocks_for(jobs, 4), 256, half, dz, out_p, out, h_prev, k_in, n, chunk, dw, db);
-    return launched("dense backward (weights)");
-}
-
-nb_status mlp_launch_adam(float *w, float *m, float *v, const float *g, uint32_t n, float lr, float b1, float b2,
-                          float eps, float c1, float c2, uint32_t *scale_state, hipStream_t s) {
+nb_status mlp_launch_adam_refresh(bool half, float *w, float *m, float *v, const float *g, uint32_t n, float lr, float b1,
+                                  float b2, float eps, float c1, float c2, uint32_t *scale_state, const MlpRefresh &r,
+                                  uint16_t *wbf, float *b5, hipStream_t s) {
     const float *scale = scale_state ? reinterpret_cast<const float *>(scale_state) : nullptr;
     const uint32_t *overflow = scale_state ? scale_state + 2 : nullptr;
     if (scale_state) {
@@ -465,8 +489,8 @@ nb_status mlp_launch_adam(float *w, float *m, float *v, const float *g, uint32_t
         nb_status st = launched("gradient check");
         if (st != NB_OK) return st;
     }
-    hipLaunchKernelGGL(adam_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, s, w, m, v, g, n, lr, b1, b2, eps, c1, c2,
-                       scale, overflow);
+    NB_MLP_LAUNCH(adam_refresh_kernel, blocks_for(n, 256), 256, half, w, m, v, g, n, lr, b1, b2, eps, c1, c2, scale,
+                  overflow, r, wbf, b5);
     nb_status st = launched("adam");
     if (st != NB_OK || !scale_state) return st;
     hipLaunchKernelGGL(scale_update_kernel, dim3(1), dim3(64), 0, s, scale_state);

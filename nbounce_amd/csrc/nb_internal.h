@@ -258,6 +258,36 @@ struct MlpDev {
     uint32_t half;                   // operands are fp16 (else bf16)
 };
 
+// One training step's rows (16-bit, row-major): encoded inputs, hidden activations, the output
+// gradient (padded to 16 channels) and the backward pass's data gradients dz_L (of layer L's
+// pre-activation); plus the targets, the loss sum and the fp16 loss scale (null for bf16).
+struct MlpTrainRows {
+    const float *target;          // n x 3
+    uint16_t *x;                  // n x 96
+    uint16_t *h[5];               // n x 64 each
+    uint16_t *dy;                 // n x 16
+    uint16_t *dz[5];              // n x 64 each
+    float *loss;
+    const float *loss_scale;
+};
+
+// The weight-gradient jobs of all layers: layer L's (out_p / 16) x (k_in / 16) tiles times the
+// row chunks are jobs job_begin[L] .. job_begin[L + 1] - 1.
+struct MlpGradJobs {
+    const uint16_t *dz[kMlpLayers];       // layer L's output gradient rows (dy for the last layer)
+    const uint16_t *h_prev[kMlpLayers];   // layer L's input rows (x for the first layer)
+    float *dw[kMlpLayers], *db[kMlpLayers];
+    uint32_t out_p[kMlpLayers], out_valid[kMlpLayers], k_in[kMlpLayers];
+    uint32_t job_begin[kMlpLayers + 1];
+    uint32_t rows_per_chunk;
+};
+
+// Where each layer's weights sit in the fp32 parameters and in the 16-bit copy, and the last
+// layer's bias (copied to the padded bias).
+struct MlpRefresh {
+    uint32_t w_off[kMlpLayers], w_size[kMlpLayers], wb_off[kMlpLayers], b5_off;
+};
+
 // One frame of cache-assisted rendering plus the training paths traced beside it.
 struct CacheFrame {
     DevScene scene;
@@ -295,18 +325,14 @@ nb_status mlp_launch_infer_count(const MlpDev &m, const float *raw, const uint32
                                  hipStream_t s);
 nb_status mlp_launch_encode(const MlpDev &m, const float *raw, uint32_t n, uint16_t *x, hipStream_t s);
 nb_status mlp_launch_infer(const MlpDev &m, const float *raw, uint32_t n, float *rgb, hipStream_t s);
-nb_status mlp_launch_dense_fwd(bool half, const uint16_t *h_in, uint32_t k_in, const uint16_t *w, const float *b,
-                               uint32_t out, uint32_t n, float *z, uint16_t *h_out, int relu, hipStream_t s);
-nb_status mlp_launch_loss(bool half, const float *y, const float *t, uint32_t n, float *dy, uint16_t *dy_bf,
-                          float *loss, const float *loss_scale, hipStream_t s);
-nb_status mlp_launch_dense_bwd_data(bool half, const uint16_t *dz, uint32_t out, const uint16_t *w, uint32_t k_in,
-                                    uint32_t n, const uint16_t *h_prev, uint16_t *dz_prev, hipStream_t s);
-nb_status mlp_launch_dense_bwd_weight(bool half, const uint16_t *dz, uint32_t out, const uint16_t *h_prev,
-                                      uint32_t k_in, uint32_t n, float *dw, float *db, hipStream_t s);
-// scale_state: the loss-scale state of an fp16 cache ([0] scale bits, [1] clean steps,
-// [2] overflow flag), or null for bf16 (no scaling)
-nb_status mlp_launch_adam(float *w, float *m, float *v, const float *g, uint32_t n, float lr, float b1, float b2,
-                          float eps, float c1, float c2, uint32_t *scale_state, hipStream_t s);
 nb_status mlp_launch_to16(bool half, const float *src, uint16_t *dst, uint32_t n, hipStream_t s);
+// One training step in four launches: forward with loss (stores tr's rows), backward through the
+// data path, all weight gradients (added to j's dw / db), Adam with the 16-bit refresh.
+nb_status mlp_launch_train_forward(const MlpDev &m, const float *raw, uint32_t n, const MlpTrainRows &tr, hipStream_t s);
+nb_status mlp_launch_backward(const MlpDev &m, uint32_t n, const MlpTrainRows &tr, hipStream_t s);
+nb_status mlp_launch_weight_grad(bool half, const MlpGradJobs &j, uint32_t n, hipStream_t s);
+nb_status mlp_launch_adam_refresh(bool half, float *w, float *m, float *v, const float *g, uint32_t n, float lr, float b1,
+                                  float b2, float eps, float c1, float c2, uint32_t *scale_state, const MlpRefresh &r,
+                                  uint16_t *wbf, float *b5, hipStream_t s);
 nb_status mlp_launch_fill(float *p, uint32_t n, float v, hipStream_t s);
 }  // namespace nb
