@@ -134,6 +134,17 @@ constexpr int kPad = 4;                 // LDS row padding (bf16) against bank c
 constexpr int kS0 = kMlpIn + kPad;      // 100
 constexpr int kSH = kMlpHidden + kPad;  // 68
 
+// Copies a (rows x cols) row-major 16-bit matrix from global memory into LDS rows of `pitch`
+// values, 4 values (8 bytes) per access: cols, pitch and the source offset are multiples of 4,
+// and the LDS array is 8-byte aligned.
+__device__ __forceinline__ void stage_rows(uint16_t *dst, const uint16_t *src, int rows, int cols, int pitch) {
+    const int q = cols / 4;
+    for (int i = threadIdx.x; i < rows * q; i += blockDim.x) {
+        const int r = i / q, c = (i % q) * 4;
+        *reinterpret_cast<uint2 *>(dst + r * pitch + c) = *reinterpret_cast<const uint2 *>(src + r * cols + c);
+    }
+}
+
 // 4 consecutive 16-bit values (8-byte aligned) to global memory
 __device__ __forceinline__ void store4(uint16_t *p, s4 v) {
     *reinterpret_cast<uint2 *>(p) = make_uint2((static_cast<uint32_t>(static_cast<uint16_t>(v[0]))) |
@@ -152,17 +163,13 @@ template <bool kHalf, bool kTrain>
 __global__ __launch_bounds__(256) void mlp_forward_kernel(MlpDev m, const float *raw, uint32_t n_max, const uint32_t *n_dev,
                                                           float *rgb, MlpTrainRows tr) {
     const uint32_t n = n_dev ? min(*n_dev, n_max) : n_max;
-    __shared__ uint16_t sw0[kMlpHidden * kS0];
-    __shared__ uint16_t swh[4][kMlpHidden * kSH];
-    __shared__ uint16_t sw5[kMlpOutPad * kSH];
+    __shared__ __attribute__((aligned(16))) uint16_t sw0[kMlpHidden * kS0];
+    __shared__ __attribute__((aligned(16))) uint16_t swh[4][kMlpHidden * kSH];
+    __shared__ __attribute__((aligned(16))) uint16_t sw5[kMlpOutPad * kSH];
     __shared__ float sb[5 * kMlpHidden + kMlpOutPad];
-    for (int i = threadIdx.x; i < kMlpHidden * kMlpIn; i += blockDim.x)
-        sw0[(i / kMlpIn) * kS0 + i % kMlpIn] = m.w[0][i];
-    for (int L = 0; L < 4; ++L)
-        for (int i = threadIdx.x; i < kMlpHidden * kMlpHidden; i += blockDim.x)
-            swh[L][(i / kMlpHidden) * kSH + i % kMlpHidden] = m.w[L + 1][i];
-    for (int i = threadIdx.x; i < kMlpOutPad * kMlpHidden; i += blockDim.x)
-        sw5[(i / kMlpHidden) * kSH + i % kMlpHidden] = m.w[5][i];
+    stage_rows(sw0, m.w[0], kMlpHidden, kMlpIn, kS0);
+    for (int L = 0; L < 4; ++L) stage_rows(swh[L], m.w[L + 1], kMlpHidden, kMlpHidden, kSH);
+    stage_rows(sw5, m.w[5], kMlpOutPad, kMlpHidden, kSH);
     for (int i = threadIdx.x; i < 5 * kMlpHidden + kMlpOutPad; i += blockDim.x)
         sb[i] = i < 5 * kMlpHidden ? m.b[i / kMlpHidden][i % kMlpHidden] : m.b[5][i - 5 * kMlpHidden];
     __syncthreads();
@@ -238,8 +245,15 @@ __global__ __launch_bounds__(256) void mlp_forward_kernel(MlpDev m, const float 
         }
     }
     if constexpr (kTrain) {
+        // one atomic per workgroup: the waves' sums meet in LDS
+        __shared__ float s_loss[4];
         for (int o = 32; o > 0; o >>= 1) loss_part += __shfl_xor(loss_part, o, 64);
-        if (lane == 0 && loss_part != 0.0f) atomicAdd(tr.loss, loss_part * (1.0f / (3.0f * static_cast<float>(n))));
+        if (lane == 0) s_loss[wave] = loss_part;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            const float sum = (s_loss[0] + s_loss[1]) + (s_loss[2] + s_loss[3]);
+            if (sum != 0.0f) atomicAdd(tr.loss, sum * (1.0f / (3.0f * static_cast<float>(n))));
+        }
     }
 }
 
@@ -251,13 +265,10 @@ __global__ __launch_bounds__(256) void mlp_forward_kernel(MlpDev m, const float 
 // masked and rounded once.
 template <bool kHalf>
 __global__ __launch_bounds__(256) void mlp_backward_kernel(MlpDev m, uint32_t n, MlpTrainRows tr) {
-    __shared__ uint16_t swh[4][kMlpHidden * kSH];
-    __shared__ uint16_t sw5[kMlpOutPad * kSH];
-    for (int L = 0; L < 4; ++L)
-        for (int i = threadIdx.x; i < kMlpHidden * kMlpHidden; i += blockDim.x)
-            swh[L][(i / kMlpHidden) * kSH + i % kMlpHidden] = m.w[L + 1][i];
-    for (int i = threadIdx.x; i < kMlpOutPad * kMlpHidden; i += blockDim.x)
-        sw5[(i / kMlpHidden) * kSH + i % kMlpHidden] = m.w[5][i];
+    __shared__ __attribute__((aligned(16))) uint16_t swh[4][kMlpHidden * kSH];
+    __shared__ __attribute__((aligned(16))) uint16_t sw5[kMlpOutPad * kSH];
+    for (int L = 0; L < 4; ++L) stage_rows(swh[L], m.w[L + 1], kMlpHidden, kMlpHidden, kSH);
+    stage_rows(sw5, m.w[5], kMlpOutPad, kMlpHidden, kSH);
     __syncthreads();
 
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -464,13 +475,13 @@ nb_status mlp_launch_infer_count(const MlpDev &m, const float *raw, const uint32
 }
 
 nb_status mlp_launch_train_forward(const MlpDev &m, const float *raw, uint32_t n, const MlpTrainRows &tr, hipStream_t s) {
-    NB_MLP_LAUNCH2(mlp_forward_kernel, true, blocks_for(n, 128, 768), 256, m.half, m, raw, n,
+    NB_MLP_LAUNCH2(mlp_forward_kernel, true, blocks_for(n, 64, 1024), 256, m.half, m, raw, n,
                    static_cast<const uint32_t *>(nullptr), static_cast<float *>(nullptr), tr);
     return launched("mlp training forward");
 }
 
 nb_status mlp_launch_backward(const MlpDev &m, uint32_t n, const MlpTrainRows &tr, hipStream_t s) {
-    NB_MLP_LAUNCH(mlp_backward_kernel, blocks_for(n, 128, 768), 256, m.half, m, n, tr);
+    NB_MLP_LAUNCH(mlp_backward_kernel, blocks_for(n, 64, 1024), 256, m.half, m, n, tr);
     return launched("mlp backward");
 }
 
