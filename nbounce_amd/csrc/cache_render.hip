@@ -8,7 +8,11 @@ namespace nb {
 namespace {
 using namespace dev;
 
-constexpr int kCK = 16;
+// LDS stack entries per lane: the query kernel's paths end after two or three traces, and an
+// 8-entry stack (4 KB per wave) with 80 VGPRs lets 6 waves share a SIMD (16 entries: 5 waves,
+// 11 % slower); the training paths are full paths and keep 16.
+constexpr int kQueryK = 8;
+constexpr int kTrainK = 16;
 
 // Shading-point features of a non-emissive hit (the same expressions as the shading in
 // bounce_step): position, outgoing direction, normal, roughness, albedo, metallic.
@@ -58,15 +62,15 @@ __device__ __forceinline__ uint32_t wave_prefix_alloc(bool want, uint32_t *count
 // after a bounce whose lobe was not a near mirror (GGX alpha >= mirror_alpha) or until bounce
 // max_cache_bounce; there it stops and queues a cache query with its throughput.  Radiance met
 // before (sky, emitters) goes to `direct`.
-__global__ __launch_bounds__(64) void cache_query_kernel(CacheFrame p) {
-    __shared__ unsigned long long s_stack[kCK * 64];
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(6))) void cache_query_kernel(CacheFrame p) {
+    __shared__ unsigned long long s_stack[kQueryK * 64];
     const uint32_t tile = (blockIdx.x & 7u) * p.tiles_per_xcd + (blockIdx.x >> 3);
     if (tile >= p.n_tiles) return;
     const uint32_t lane = threadIdx.x;
     const uint32_t x = (tile % p.tiles_x) * 8u + (lane & 7u), y = (tile / p.tiles_x) * 8u + (lane >> 3);
     const uint32_t pixel = y * p.width + x;
     SpillArray spill;
-    Stack<kCK> st;
+    Stack<kQueryK> st;
     st.init(s_stack, lane, spill);
     LaneCounters c{0, 0, 0, 0, 0};
     PathConst pc;
@@ -86,7 +90,7 @@ __global__ __launch_bounds__(64) void cache_query_kernel(CacheFrame p) {
     for (uint32_t bounce = 0; bounce <= p.bounces; ++bounce) {
         if (bounce >= 1 && (prev_alpha >= p.mirror_alpha || bounce >= p.max_cache_bounce)) {
             Hit h;
-            trace_counted<kCK>(pc, ray, h, st, c);
+            trace_counted<kQueryK>(pc, ray, h, st, c);
             if (h.t == kNoHit) { out = mul(thr, env_sample(pc.env, ray.d)); break; }
             F3 emit;
             if (!hit_vertex(pc.scene, ray, h, raw, emit)) { out = mul(thr, emit); break; }
@@ -94,7 +98,7 @@ __global__ __launch_bounds__(64) void cache_query_kernel(CacheFrame p) {
             break;
         }
         Vertex v;
-        if (bounce_step<kCK, true>(pc, bounce, ray, thr, out, st, c, &v) == kDone) break;
+        if (bounce_step<kQueryK, true>(pc, bounce, ray, thr, out, st, c, &v) == kDone) break;
         prev_alpha = v.lobe_alpha;
     }
     p.direct[pixel] = make_float4(out.x, out.y, out.z, 1.0f);
@@ -111,10 +115,10 @@ __global__ __launch_bounds__(64) void cache_query_kernel(CacheFrame p) {
 // a training record whose target is the path's final radiance divided by the throughput that
 // arrived at that point, i.e. the one-sample estimate of the point's outgoing radiance.
 __global__ __launch_bounds__(64) void cache_train_paths_kernel(CacheFrame p) {
-    __shared__ unsigned long long s_stack[kCK * 64];
+    __shared__ unsigned long long s_stack[kTrainK * 64];
     const uint32_t lane = threadIdx.x;
     SpillArray spill;
-    Stack<kCK> st;
+    Stack<kTrainK> st;
     st.init(s_stack, lane, spill);
     LaneCounters c{0, 0, 0, 0, 0};
     const uint32_t npix = p.width * p.height;
@@ -143,7 +147,7 @@ __global__ __launch_bounds__(64) void cache_train_paths_kernel(CacheFrame p) {
             F3 thr = f3(1.0f, 1.0f, 1.0f);
             for (uint32_t bounce = 0; bounce <= p.bounces; ++bounce) {
                 Hit h;
-                trace_counted<kCK>(pc, ray, h, st, c);
+                trace_counted<kTrainK>(pc, ray, h, st, c);
                 if (h.t == kNoHit) { out = mul(thr, env_sample(pc.env, ray.d)); break; }
                 F3 emit;
                 float raw[kMlpRaw];
