@@ -298,6 +298,7 @@ def cache_bench(args):
         return rel_mse(img[xy[:, 1], xy[:, 0]], ref_mean)
 
     cache = RadianceCache(ctx, [0, 0, 0], [10, 10, 10], seed=0, max_batch=1 << 16)
+    ctx.enable_render_timing()
     train_ms, losses = [], []
     plain_orbit_ms = []
     for k in range(P["frames"]):

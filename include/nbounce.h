@@ -205,7 +205,13 @@ nb_status nb_render(nb_context *ctx, const nb_scene *scene, nb_target *target, c
 nb_status nb_render_frames(nb_context *ctx, const nb_scene *scene, nb_target *target, const nb_camera *camera,
                            const nb_globals *first, uint32_t n_frames, const nb_envmap *env);
 
-/* Device time of the last render call in milliseconds (HIP events on the context stream). */
+/* Per-render timing, off by default: with it on, nb_render, nb_render_frames and
+ * nb_render_cached record HIP events around their launches on the context stream (the records
+ * between launches cost a frame stream about 0.5 %).  No counterpart in the reference, which has
+ * no timing API. */
+nb_status nb_render_timing(nb_context *ctx, uint32_t on);
+/* Device time of the last render call in milliseconds (its HIP events); NB_ERR_INVALID_ARGUMENT
+ * when no render was timed since per-render timing was switched on. */
 nb_status nb_last_render_ms(nb_context *ctx, float *ms);
 /* HIP events around a region of work on the context stream: start records the first event,
  * stop records the second, waits for it and returns the elapsed milliseconds. */

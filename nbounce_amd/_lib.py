@@ -54,6 +54,7 @@ SIGNATURES = {
     "nb_render": [_p, _p, _p, _p, _p, _p],
     "nb_render_frames": [_p, _p, _p, _p, _p, _u32, _p],
     "nb_last_render_ms": [_p, _ptr(_f32)],
+    "nb_render_timing": [_p, _u32],
     "nb_stream_timer_start": [_p],
     "nb_stream_timer_stop": [_p, _ptr(_f32)],
     "nb_counters_get": [_p, _p],

@@ -430,11 +430,13 @@ nb_status nb_render_cached(nb_context *ctx, const nb_scene *scene, nb_target *ta
     if (!ctx->sobol) return set_error(NB_ERR_INVALID_ARGUMENT, "no Sobol table set (nb_sobol_set)");
     if (g->sample == 0) return set_error(NB_ERR_INVALID_ARGUMENT, "samples are 1-based");
     C_HIP(hipSetDevice(ctx->device));
-    C_HIP(hipEventRecord(ctx->ev_start, ctx->stream));
+    if (ctx->render_timing) C_HIP(hipEventRecord(ctx->ev_start, ctx->stream));
     C_TRY(cache_render_frame(ctx, scene->dev, DevEnv{env->texels, env->n}, target->pixels, target->width,
                              target->height, camera->clip_to_world, *g, c, train_paths, lr, loss));
-    C_HIP(hipEventRecord(ctx->ev_stop, ctx->stream));
-    ctx->timed = true;
+    if (ctx->render_timing) {
+        C_HIP(hipEventRecord(ctx->ev_stop, ctx->stream));
+        ctx->timed = true;
+    }
     if (loss && train_paths > 0) C_HIP(hipStreamSynchronize(ctx->stream));
     return NB_OK;
 }

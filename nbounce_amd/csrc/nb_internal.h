@@ -174,6 +174,7 @@ struct nb_context {
     hipEvent_t ev_start = nullptr, ev_stop = nullptr;
     hipEvent_t ev_region_start = nullptr, ev_region_stop = nullptr;
     bool timed = false, region_started = false;
+    bool render_timing = false;               // nb_render_timing: per-render events
     float4 *sobol = nullptr;
     uint32_t sobol_entries = 0, sobol_stride = 17;
     unsigned long long *counters = nullptr;   // kCounterSlots x kCounterWords
