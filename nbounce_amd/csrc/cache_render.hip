@@ -10,7 +10,9 @@ using namespace dev;
 
 // LDS stack entries per lane: the query kernel's paths end after two or three traces, and an
 // 8-entry stack (4 KB per wave) with 80 VGPRs lets 6 waves share a SIMD (16 entries: 5 waves,
-// 11 % slower); the training paths are full paths and keep 16.
+// 11 % slower); the training paths are full paths and keep 16.  Held to 72 VGPRs for 7 waves,
+// the query kernel is 2.5 % faster alone, but the training-path kernel beside it on the side
+// stream slows down and a training frame takes 5 % longer.
 constexpr int kQueryK = 8;
 constexpr int kTrainK = 16;
 
