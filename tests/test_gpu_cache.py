@@ -140,6 +140,32 @@ def test_training_reduces_the_loss(cache):
     assert 1.0 <= scale.value <= 65536.0
 
 
+def test_fp16_overflow_skips_the_step_and_halves_the_scale(cache):
+    """A training step whose loss-scaled gradient is not finite in FP16 leaves every parameter
+    as it was (the Adam step is skipped on the device) and halves the loss scale."""
+    lib, ptr, check, h = cache[:4]
+    if _fmt(cache) != "fp16":
+        pytest.skip("loss scaling is FP16-only")
+    print("case: FP16 training step with a non-finite gradient", flush=True)
+    raw, _ = _batch(np.random.default_rng(9), 1024)
+    target = np.full((1024, 3), 1e30, dtype=np.float32)   # (y - t) / (y^2 + 0.01) overflows FP16
+    s0, s1 = ctypes.c_float(), ctypes.c_float()
+    check(lib.nb_cache_loss_scale(h, ctypes.byref(s0)))
+    p0 = _params(cache)
+    loss = ctypes.c_float()
+    check(lib.nb_cache_train(h, ptr(raw), ptr(target), 1024, 1024, 1e-3, ctypes.byref(loss)))
+    p1 = _params(cache)
+    check(lib.nb_cache_loss_scale(h, ctypes.byref(s1)))
+    print(f"loss scale {s0.value} -> {s1.value}, loss {loss.value}")
+    np.testing.assert_array_equal(p1.view(np.uint32), p0.view(np.uint32))
+    assert s1.value == max(s0.value * 0.5, 1.0)
+    # the next finite step trains again
+    raw2, target2 = _batch(np.random.default_rng(10), 1024)
+    check(lib.nb_cache_train(h, ptr(raw2), ptr(target2), 1024, 1024, 1e-3, ctypes.byref(loss)))
+    assert np.isfinite(loss.value)
+    assert not np.array_equal(_params(cache), p0)
+
+
 @pytest.mark.parametrize("fp16", [False, True])
 def test_trained_cache_beats_the_plain_tracer_at_one_sample(fp16):
     """What the trained network gains: at one sample per pixel the cache-assisted image is
