@@ -182,6 +182,25 @@ def test_shared_blas_layout_matches_oracle_and_reference_layout(nb, ctx, large_g
     assert same > 0.95
 
 
+def test_render_timing_is_opt_in(nb, ctx, small_gpu):
+    """Per-render HIP events are recorded only after nb_render_timing(ctx, 1); before that
+    nb_last_render_ms reports that no render was timed."""
+    w, sb, env = small_gpu
+    from nbounce_amd import scenegen
+    P = scenegen.SMALL
+    print("case: per-render timing switched off, then on", flush=True)
+    pt, _ = _pathtracer(nb, ctx, sb, env, (64, 64), P["eye"], P["target"])
+    ctx.enable_render_timing(False)
+    pt.dispatch()
+    pt.output_texture()
+    with pytest.raises(nb.NbError):
+        ctx.last_render_ms()
+    ctx.enable_render_timing(True)
+    pt.dispatch()
+    assert ctx.last_render_ms() > 0.0
+    ctx.enable_render_timing(False)
+
+
 def test_pathtracer_surface_caps_and_invalidates(nb, ctx, small_gpu):
     """Pathtracer::dispatch stops at max_sample_count (pathtracer.rs:226) and invalidate()
     restarts accumulation at sample 1 (pathtracer.rs:221-223), overwriting the old image."""
