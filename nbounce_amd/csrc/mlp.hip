@@ -130,6 +130,9 @@ __device__ __forceinline__ void feature4(const float *row, int f0, const MlpDev 
     }
 }
 
+// Threads per workgroup of the forward kernel: 8 waves share one LDS copy of the weights (51 KB),
+// so two workgroups and 4 waves per SIMD fit a CU (with 128 VGPRs).
+constexpr uint32_t kFwdThreads = 512;
 constexpr int kPad = 4;                 // LDS row padding (bf16) against bank conflicts
 constexpr int kS0 = kMlpIn + kPad;      // 100
 constexpr int kSH = kMlpHidden + kPad;  // 68
@@ -160,7 +163,7 @@ __device__ __forceinline__ void store4(uint16_t *p, s4 v) {
 // the output gradient 2 (y - t) / (y^2 + 0.01) / (3n), times the loss scale and rounded to
 // 16 bits, padded to 16 channels with zeros.
 template <bool kHalf, bool kTrain>
-__global__ __launch_bounds__(256) void mlp_forward_kernel(MlpDev m, const float *raw, uint32_t n_max, const uint32_t *n_dev,
+__global__ __launch_bounds__(kFwdThreads) __attribute__((amdgpu_waves_per_eu(4))) void mlp_forward_kernel(MlpDev m, const float *raw, uint32_t n_max, const uint32_t *n_dev,
                                                           float *rgb, MlpTrainRows tr) {
     const uint32_t n = n_dev ? min(*n_dev, n_max) : n_max;
     __shared__ __attribute__((aligned(16))) uint16_t sw0[kMlpHidden * kS0];
@@ -176,9 +179,9 @@ __global__ __launch_bounds__(256) void mlp_forward_kernel(MlpDev m, const float 
 
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int col = lane & 15, kq = lane >> 4;
-    const uint32_t waves = gridDim.x * 4u;
+    const uint32_t per_block = kFwdThreads / 64u, waves = gridDim.x * per_block;
     float loss_part = 0.0f;
-    for (uint32_t base = (blockIdx.x * 4u + wave) * 16u; base < n; base += waves * 16u) {
+    for (uint32_t base = (blockIdx.x * per_block + wave) * 16u; base < n; base += waves * 16u) {
         const uint32_t row = base + col;
         // rows past n compute the features of row n - 1; their outputs are not stored
         const float *rowp = raw + static_cast<size_t>(row < n ? row : n - 1u) * kMlpRaw;
@@ -246,12 +249,13 @@ __global__ __launch_bounds__(256) void mlp_forward_kernel(MlpDev m, const float 
     }
     if constexpr (kTrain) {
         // one atomic per workgroup: the waves' sums meet in LDS
-        __shared__ float s_loss[4];
+        __shared__ float s_loss[kFwdThreads / 64];
         for (int o = 32; o > 0; o >>= 1) loss_part += __shfl_xor(loss_part, o, 64);
         if (lane == 0) s_loss[wave] = loss_part;
         __syncthreads();
         if (threadIdx.x == 0) {
-            const float sum = (s_loss[0] + s_loss[1]) + (s_loss[2] + s_loss[3]);
+            float sum = 0.0f;
+            for (uint32_t k = 0; k < kFwdThreads / 64u; ++k) sum += s_loss[k];
             if (sum != 0.0f) atomicAdd(tr.loss, sum * (1.0f / (3.0f * static_cast<float>(n))));
         }
     }
@@ -462,20 +466,20 @@ nb_status mlp_launch_encode(const MlpDev &m, const float *raw, uint32_t n, uint1
 }
 
 nb_status mlp_launch_infer(const MlpDev &m, const float *raw, uint32_t n, float *rgb, hipStream_t s) {
-    NB_MLP_LAUNCH2(mlp_forward_kernel, false, blocks_for(n, 64, 768), 256, m.half, m, raw, n,
+    NB_MLP_LAUNCH2(mlp_forward_kernel, false, blocks_for(n, kFwdThreads / 4, 512), kFwdThreads, m.half, m, raw, n,
                    static_cast<const uint32_t *>(nullptr), rgb, MlpTrainRows{});
     return launched("mlp infer");
 }
 
 nb_status mlp_launch_infer_count(const MlpDev &m, const float *raw, const uint32_t *n_dev, uint32_t n_max, float *rgb,
                                  hipStream_t s) {
-    NB_MLP_LAUNCH2(mlp_forward_kernel, false, blocks_for(n_max, 64, 768), 256, m.half, m, raw, n_max, n_dev, rgb,
+    NB_MLP_LAUNCH2(mlp_forward_kernel, false, blocks_for(n_max, kFwdThreads / 4, 512), kFwdThreads, m.half, m, raw, n_max, n_dev, rgb,
                    MlpTrainRows{});
     return launched("mlp infer");
 }
 
 nb_status mlp_launch_train_forward(const MlpDev &m, const float *raw, uint32_t n, const MlpTrainRows &tr, hipStream_t s) {
-    NB_MLP_LAUNCH2(mlp_forward_kernel, true, blocks_for(n, 64, 1024), 256, m.half, m, raw, n,
+    NB_MLP_LAUNCH2(mlp_forward_kernel, true, blocks_for(n, kFwdThreads / 4, 1024), kFwdThreads, m.half, m, raw, n,
                    static_cast<const uint32_t *>(nullptr), static_cast<float *>(nullptr), tr);
     return launched("mlp training forward");
 }
