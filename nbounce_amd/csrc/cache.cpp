@@ -113,8 +113,7 @@ nb_status cache_forward_backward(nb_cache *c, const float *raw, const float *tar
     for (int L = 0; L < 5; ++L) { tr.h[L] = c->h[L]; tr.dz[L] = c->dz[L]; }
     tr.loss = c->loss;
     tr.loss_scale = c->half ? reinterpret_cast<const float *>(c->scale_state) : nullptr;
-    C_TRY(mlp_launch_fill(c->loss, 1, 0.0f, s));
-    C_TRY(mlp_launch_fill(c->grad, kMlpParams, 0.0f, s));
+    C_TRY(mlp_launch_fill(c->grad, kMlpParams + 1, 0.0f, s));   // the gradient and the loss after it
     C_TRY(mlp_launch_train_forward(m, raw, n, tr, s));
     C_TRY(mlp_launch_backward(m, n, tr, s));
     // layer L's weight gradient: its output gradient rows (dy for the last layer) against its
@@ -196,9 +195,11 @@ nb_status nb_cache_create_ex(nb_context *ctx, const float scene_min[3], const fl
     if (e == hipSuccess) e = hipHostMalloc(&c->h_count, 16, hipHostMallocDefault);
     auto alloc = [&](auto **p, size_t bytes) { if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void **>(p), bytes); };
     alloc(&c->params, kMlpParams * 4); alloc(&c->adam_m, kMlpParams * 4); alloc(&c->adam_v, kMlpParams * 4);
-    alloc(&c->grad, kMlpParams * 4); alloc(&c->wbf, kBf16Weights * 2); alloc(&c->b5, kMlpOutPad * 4);
+    // the loss sum is the word after the gradient, so one fill clears both before a step
+    alloc(&c->grad, (kMlpParams + 1) * 4); alloc(&c->wbf, kBf16Weights * 2); alloc(&c->b5, kMlpOutPad * 4);
     alloc(&c->raw, B * kMlpRaw * 4); alloc(&c->target, B * 3 * 4); alloc(&c->rgb, B * 3 * 4);
-    alloc(&c->loss, 4); alloc(&c->x, B * kMlpIn * 2); alloc(&c->dy, B * kMlpOutPad * 2);
+    if (e == hipSuccess) c->loss = c->grad + kMlpParams;
+    alloc(&c->x, B * kMlpIn * 2); alloc(&c->dy, B * kMlpOutPad * 2);
     alloc(&c->counts, 16);
     alloc(&c->scale_state, 16);
     for (int L = 0; L < 5; ++L) { alloc(&c->h[L], B * kMlpHidden * 2); alloc(&c->dz[L], B * kMlpHidden * 2); }
@@ -240,7 +241,7 @@ nb_status nb_cache_destroy(nb_cache *c) {
     if (c->ev_fork) hipEventDestroy(c->ev_fork);
     if (c->ev_join) hipEventDestroy(c->ev_join);
     if (c->h_count) hipHostFree(c->h_count);
-    void *ps[] = {c->params, c->adam_m, c->adam_v, c->grad, c->wbf, c->b5, c->raw, c->target, c->rgb, c->loss,
+    void *ps[] = {c->params, c->adam_m, c->adam_v, c->grad, c->wbf, c->b5, c->raw, c->target, c->rgb,
                   c->x, c->dy, c->direct, c->q_raw, c->q_thr, c->q_rgb, c->counts, c->scale_state};
     for (void *p : ps) hipFree(p);
     for (int L = 0; L < 5; ++L) { hipFree(c->h[L]); hipFree(c->dz[L]); }
