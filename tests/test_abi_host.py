@@ -22,6 +22,18 @@ def _declared_symbols():
     return sorted(set(re.findall(r"\b(nb_[a-z0-9_]+)\s*\(", text)))
 
 
+def test_package_refuses_to_import_without_the_library(tmp_path):
+    """The library is the only compute path: with no libnbounce.so to load, importing the
+    package fails (there is no CPU fallback)."""
+    import subprocess
+    import sys
+    env = dict(os.environ, NB_LIBRARY=str(tmp_path / "absent" / "libnbounce.so"))
+    r = subprocess.run([sys.executable, "-c", "import nbounce_amd"], cwd=ROOT, env=env, capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode != 0
+    assert "libnbounce.so" in r.stderr
+
+
 def test_library_exports_every_declared_symbol():
     names = _declared_symbols()
     assert len(names) >= 25
