@@ -32,6 +32,8 @@ def common_flags():
         "-ffp-contract=off", "-fhip-fp32-correctly-rounded-divide-sqrt", "-fno-gpu-flush-denormals-to-zero",
         "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
         "-Wall", "-Wno-unused-function", "-Wno-unused-value", "-Wno-unused-result",
+        # a value read before it is written is an error in host and device code alike
+        "-Werror=uninitialized", "-Werror=sometimes-uninitialized", "-Werror=conditional-uninitialized",
         "-I", os.path.join(ROOT, "include"), "-I", CSRC,
     ]
 

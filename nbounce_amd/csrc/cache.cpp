@@ -431,6 +431,17 @@ nb_status cache_render_frame(nb_context *ctx, const DevScene &scene, const DevEn
 
 nb_context *cache_context(const nb_cache *c) { return c->ctx; }
 
+uint64_t cache_workspace_bytes(uint64_t pixels) { return pixels * (16 + kMlpRaw * 4 + 16 + 12); }
+
+uint64_t cache_workspace_pixels(const nb_cache *c) { return c->ws_pixels; }
+
+uint64_t cache_device_bytes(const nb_cache *c) {
+    const uint64_t B = c->max_batch;
+    const uint64_t fixed = 3ull * kMlpParams * 4 + (kMlpParams + 1) * 4 + kBf16Weights * 2 + kMlpOutPad * 4 + 16 + 16;
+    const uint64_t batch = B * kMlpRaw * 4 + B * 3 * 4 * 2 + B * kMlpIn * 2 + B * kMlpOutPad * 2 + 10ull * B * kMlpHidden * 2;
+    return fixed + batch + cache_workspace_bytes(c->ws_pixels);
+}
+
 }  // namespace nb
 
 extern "C" {

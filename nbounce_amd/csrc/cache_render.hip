@@ -75,7 +75,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(6))) void ca
     Stack<kQueryK> st;
     st.init(s_stack, lane, spill);
     LaneCounters c{0, 0, 0, 0, 0};
-    PathConst pc;
+    PathConst pc{};
     pc.scene = p.scene;
     pc.env = p.env;
     pc.sobol = SobolView{p.sobol, p.sobol_entries, p.sobol_stride};
@@ -91,7 +91,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(6))) void ca
     float raw[kMlpRaw];
     for (uint32_t bounce = 0; bounce <= p.bounces; ++bounce) {
         if (bounce >= 1 && (prev_alpha >= p.mirror_alpha || bounce >= p.max_cache_bounce)) {
-            Hit h;
+            Hit h{};
             trace_counted<kQueryK>(pc, ray, h, st, c);
             if (h.t == kNoHit) { out = mul(thr, env_sample(pc.env, ray.d)); break; }
             F3 emit;
@@ -136,7 +136,7 @@ __global__ __launch_bounds__(64) void cache_train_paths_kernel(CacheFrame p) {
         if (active) {
             const uint32_t pixel = (i * stride + p.train_offset) % npix;
             const uint32_t x = pixel % p.width, y = pixel / p.width;
-            PathConst pc;
+            PathConst pc{};
             pc.scene = p.scene;
             pc.env = p.env;
             pc.sobol = SobolView{p.sobol, p.sobol_entries, p.sobol_stride};
@@ -148,7 +148,7 @@ __global__ __launch_bounds__(64) void cache_train_paths_kernel(CacheFrame p) {
             Ray ray = camera_ray(p.c2w, p.width, p.height, x, y, jit.x, jit.y);
             F3 thr = f3(1.0f, 1.0f, 1.0f);
             for (uint32_t bounce = 0; bounce <= p.bounces; ++bounce) {
-                Hit h;
+                Hit h{};
                 trace_counted<kTrainK>(pc, ray, h, st, c);
                 if (h.t == kNoHit) { out = mul(thr, env_sample(pc.env, ray.d)); break; }
                 F3 emit;

@@ -49,13 +49,13 @@ __device__ __forceinline__ f4v mfma(s4 a, s4 b, f4v c) {
 
 __device__ __forceinline__ s4 load4(const uint16_t *p) {   // 4 consecutive bf16 (8-byte aligned)
     uint2 v = *reinterpret_cast<const uint2 *>(p);
-    s4 r;
+    s4 r = {};
     r[0] = static_cast<short>(v.x & 0xFFFFu); r[1] = static_cast<short>(v.x >> 16);
     r[2] = static_cast<short>(v.y & 0xFFFFu); r[3] = static_cast<short>(v.y >> 16);
     return r;
 }
 __device__ __forceinline__ s4 load4_strided(const uint16_t *p, uint32_t stride) {
-    s4 r;
+    s4 r = {};
     for (int i = 0; i < 4; ++i) r[i] = static_cast<short>(p[i * stride]);
     return r;
 }
@@ -287,7 +287,7 @@ __global__ __launch_bounds__(256) void mlp_backward_kernel(MlpDev m, uint32_t n,
         const s4 g5 = valid ? load4(tr.dy + static_cast<size_t>(row) * kMlpOutPad + 4 * kq) : zero;
         s4 dz[4];
         for (int mt = 0; mt < 4; ++mt) {
-            s4 a;
+            s4 a = {};
             for (int i = 0; i < 4; ++i) a[i] = static_cast<short>(sw5[(4 * kq + i) * kSH + 16 * mt + col]);
             const f4v acc = mfma<kHalf>(a, g5, f4v{0.0f, 0.0f, 0.0f, 0.0f});
             const s4 hp = valid ? load4(tr.h[4] + hrow + 16 * mt + 4 * kq) : zero;
@@ -301,7 +301,7 @@ __global__ __launch_bounds__(256) void mlp_backward_kernel(MlpDev m, uint32_t n,
             for (int mt = 0; mt < 4; ++mt) {
                 f4v acc = {0.0f, 0.0f, 0.0f, 0.0f};
                 for (int t = 0; t < 4; ++t) {
-                    s4 a;
+                    s4 a = {};
                     for (int i = 0; i < 4; ++i) a[i] = static_cast<short>(swh[L - 1][(16 * t + 4 * kq + i) * kSH + 16 * mt + col]);
                     acc = mfma<kHalf>(a, dz[t], acc);
                 }
@@ -345,7 +345,7 @@ __device__ __forceinline__ void weight_grad_job(const uint16_t *dz, uint32_t out
     f4v acc = {0.0f, 0.0f, 0.0f, 0.0f};
     float bsum = 0.0f;
     for (uint32_t rb = r0; rb < r1; rb += 16) {
-        s4 a, bop;
+        s4 a = {}, bop = {};
         for (int i = 0; i < 4; ++i) {
             const uint32_t row = rb + 4 * kq + i;
             const bool ok = row < r1;

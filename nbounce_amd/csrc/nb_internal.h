@@ -321,6 +321,11 @@ nb_status cache_render_frame(nb_context *ctx, const DevScene &scene, const DevEn
                              uint32_t height, const float *c2w, const nb_globals &g, nb_cache *c, uint32_t train_paths,
                              float lr, float *loss);
 nb_context *cache_context(const nb_cache *c);
+// Device bytes a cache holds (parameters, Adam state, batch buffers, frame workspace), the
+// pixels its frame workspace currently covers, and the workspace bytes a frame of n pixels needs.
+uint64_t cache_device_bytes(const nb_cache *c);
+uint64_t cache_workspace_pixels(const nb_cache *c);
+uint64_t cache_workspace_bytes(uint64_t pixels);
 nb_status cache_launch_train_paths(const CacheFrame &p, hipStream_t s);
 nb_status mlp_launch_infer_count(const MlpDev &m, const float *raw, const uint32_t *n_dev, uint32_t n_max, float *rgb,
                                  hipStream_t s);

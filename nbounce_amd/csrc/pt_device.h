@@ -353,7 +353,7 @@ enum LaneMode : uint32_t { kIdle = 0, kTracing = 1, kReady = 2 };
 // raytracing_sw.wgsl:176-319).
 template <int K>
 __device__ __forceinline__ void trace(const DevScene &S, const Ray &world, Hit &hit, Stack<K> &st, LaneCounters &c) {
-    TraceState ts;
+    TraceState ts{};
     trace_begin<K>(S, world, ts, st);
     while (!trace_step<K>(S, world, ts, st, c)) {}
     hit = ts.hit;
@@ -589,10 +589,10 @@ __device__ StepResult shade(const PathConst &pc, uint32_t bounce, const Hit &h, 
 template <int K, bool kRecord = false>
 __device__ __forceinline__ StepResult bounce_step(const PathConst &pc, uint32_t bounce, Ray &ray, F3 &thr, F3 &out,
                                                   Stack<K> &st, LaneCounters &c, Vertex *vtx = nullptr) {
-    Hit h;
+    Hit h{};
     trace_counted<K>(pc, ray, h, st, c);
     if (h.t == kNoHit) { out = mul(thr, env_sample(pc.env, ray.d)); return kDone; }
-    Vertex dummy;
+    Vertex dummy{};
     return shade<kRecord>(pc, bounce, h, ray, thr, out, kRecord ? *vtx : dummy);
 }
 

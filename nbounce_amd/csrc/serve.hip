@@ -93,7 +93,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5))) void se
     uint32_t g = __builtin_amdgcn_readfirstlane(
         group_of_unit(S, static_cast<uint32_t>(static_cast<uint64_t>(blockIdx.x) * S.n_units / gridDim.x)));
     DevScene scene = uniform_scene(S.scenes[gp(S.groups + g)->scene].scene);
-    PathConst pc;
+    PathConst pc{};
     pc.scene = scene;
     pc.env = S.scenes[gp(S.groups + g)->scene].env;
     pc.sobol = SobolView{S.sobol, S.sobol_entries, S.sobol_stride};
@@ -104,8 +104,9 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5))) void se
     uint32_t mode = kIdle;
     uint32_t x = 0, y = 0, bounce = 0, samples_done = 0;
     uint32_t path = 0;   // the lane's path id
-    Ray ray;
-    TraceState ts;
+    // every per-lane value starts defined: idle lanes take part in the wave's ballots
+    Ray ray{};
+    TraceState ts{};
     F3 thr = f3(1.0f, 1.0f, 1.0f);
     for (;;) {
         bool start = false;
@@ -114,13 +115,13 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5))) void se
             const Hit &h = ts.hit;
             ++c.rays;
             F3 out = f3(0.0f, 0.0f, 0.0f);
-            StepResult r;
+            StepResult r = kDone;
             if (h.t == kNoHit) {
                 out = mul(thr, env_sample(pc.env, ray.d));
                 r = kDone;
             } else {
                 ++c.hits;
-                Vertex dummy;
+                Vertex dummy{};
                 pc.shift = hash4f(x, y, x, y);
                 r = shade<false>(pc, bounce, h, ray, thr, out, dummy);
                 if (r == kContinue && bounce == pc.bounces) { r = kDone; out = f3(0.0f, 0.0f, 0.0f); }

@@ -18,6 +18,8 @@
 #include <cstring>
 #include <deque>
 #include <map>
+#include <new>
+#include <set>
 #include <vector>
 
 using namespace nb;
@@ -57,7 +59,12 @@ struct nb_server {
     nb_context *ctx = nullptr;
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
-    uint64_t budget = 0, in_use = 0;
+    // HBM admission: in_use counts the server's own step buffers, the resident scenes and
+    // environments, the caches (parameters, Adam state, batch buffers and the frame workspace
+    // the largest admitted cached request needs) and the admitted requests' images.
+    uint64_t budget = 0, in_use = 0, resident = 0;
+    std::set<const void *> counted;             // scenes, environments and caches already charged
+    std::map<const nb_cache *, uint64_t> cache_ws_pixels;   // frame workspace charged per cache
     uint32_t max_paths = 0, max_bounces = 0, cu = 0;
     std::chrono::steady_clock::time_point t0;
 
@@ -153,9 +160,22 @@ nb_status admit(nb_server *s) {
         Request *r = s->queued.front();
         int slot = -1;
         for (uint32_t k = 0; k < kMaxSlots; ++k) if (!s->slots[k]) { slot = static_cast<int>(k); break; }
-        if (slot < 0 || s->in_use + r->bytes > s->budget) break;   // FIFO: wait for slots / memory
+        // a cached request may grow its cache's frame workspace (kept for the server's life)
+        uint64_t ws = 0, ws_px = 0;
+        nb_cache *cache = r->req.mode == NB_REQUEST_CACHED ? s->caches[r->req.scene] : nullptr;
+        if (cache) {
+            ws_px = static_cast<uint64_t>(r->req.width) * r->req.height;
+            const uint64_t have = s->cache_ws_pixels[cache];
+            ws = ws_px > have ? cache_workspace_bytes(ws_px) - cache_workspace_bytes(have) : 0;
+        }
+        if (slot < 0 || s->in_use + r->bytes + ws > s->budget) break;   // FIFO: wait for slots / memory
         SRV_HIP(hipMalloc(&r->target, r->bytes));
         SRV_HIP(hipMemsetAsync(r->target, 0, r->bytes, s->stream));
+        if (ws) {
+            s->cache_ws_pixels[cache] = ws_px;
+            s->resident += ws;
+            s->in_use += ws;
+        }
         s->in_use += r->bytes;
         r->slot = slot;
         r->admit = s->now();
@@ -216,6 +236,16 @@ nb_status nb_server_create(nb_context *ctx, uint64_t budget, uint32_t max_paths,
         return set_error(e == hipErrorOutOfMemory ? NB_ERR_OUT_OF_MEMORY : NB_ERR_HIP, "server allocation: %s",
                          hipGetErrorString(e));
     }
+    // the server's own device buffers count against the budget
+    s->resident = kServeMaxGroups * kWorkStride * sizeof(uint32_t) + static_cast<uint64_t>(max_paths) * 16 +
+                  kCounterBytes + sizeof(ServeStep) + kMaxSlots * (sizeof(ServePass) + sizeof(ServeRequestDev)) +
+                  kServeMaxGroups * sizeof(ServeGroup) + kMaxScenes * sizeof(ServeSceneDev);
+    s->in_use = s->resident;
+    if (s->resident > budget) {
+        const unsigned long long need = s->resident;
+        free_server(s);
+        return set_error(NB_ERR_BUDGET, "the server's step buffers (%llu B) exceed the HBM budget", need);
+    }
     std::memset(s->h_requests, 0, kMaxSlots * sizeof(ServeRequestDev));
     std::memset(s->h_step, 0, sizeof(ServeStep));
     nb_status st = capture_graph(s);
@@ -234,6 +264,21 @@ nb_status nb_server_add_scene(nb_server *s, const nb_scene *scene, const nb_envm
     if (!s || !scene || !env || !index) return set_error(NB_ERR_INVALID_ARGUMENT, "null argument");
     if (scene->ctx != s->ctx || env->ctx != s->ctx) return set_error(NB_ERR_INVALID_ARGUMENT, "scene and environment must belong to the server's context");
     if (s->n_scenes >= kMaxScenes) return set_error(NB_ERR_UNSUPPORTED, "at most %u resident scenes", kMaxScenes);
+    // a scene or environment shared by several resident scenes is charged once
+    uint64_t add = 0;
+    if (!s->counted.count(scene)) add += scene->bytes;
+    if (!s->counted.count(env)) add += 6ull * (env->n + 2) * (env->n + 2) * 16;
+    if (s->in_use + add > s->budget)
+        return set_error(NB_ERR_BUDGET, "scene %u needs %llu B, %llu B of the HBM budget are left", s->n_scenes,
+                         static_cast<unsigned long long>(add), static_cast<unsigned long long>(s->budget - s->in_use));
+    try {
+        s->counted.insert(scene);
+        s->counted.insert(env);
+    } catch (const std::bad_alloc &) {
+        return set_error(NB_ERR_OUT_OF_MEMORY, "host allocation failed");
+    }
+    s->resident += add;
+    s->in_use += add;
     s->h_scenes[s->n_scenes] = ServeSceneDev{scene->dev, DevEnv{env->texels, env->n}};
     SRV_HIP(hipSetDevice(s->ctx->device));
     SRV_HIP(hipMemcpy(s->d_scenes, s->h_scenes, (s->n_scenes + 1) * sizeof(ServeSceneDev), hipMemcpyHostToDevice));
@@ -245,6 +290,19 @@ nb_status nb_server_set_cache(nb_server *s, uint32_t scene, nb_cache *cache) {
     if (!s || !cache) return set_error(NB_ERR_INVALID_ARGUMENT, "null argument");
     if (scene >= s->n_scenes) return set_error(NB_ERR_INVALID_ARGUMENT, "scene %u not added", scene);
     if (cache_context(cache) != s->ctx) return set_error(NB_ERR_INVALID_ARGUMENT, "the cache belongs to another context");
+    if (s->caches[scene]) return set_error(NB_ERR_INVALID_ARGUMENT, "scene %u already has a cache", scene);
+    const uint64_t add = s->counted.count(cache) ? 0 : cache_device_bytes(cache);
+    if (s->in_use + add > s->budget)
+        return set_error(NB_ERR_BUDGET, "the cache needs %llu B, %llu B of the HBM budget are left",
+                         static_cast<unsigned long long>(add), static_cast<unsigned long long>(s->budget - s->in_use));
+    try {
+        s->counted.insert(cache);
+        s->cache_ws_pixels[cache] = cache_workspace_pixels(cache);
+    } catch (const std::bad_alloc &) {
+        return set_error(NB_ERR_OUT_OF_MEMORY, "host allocation failed");
+    }
+    s->resident += add;
+    s->in_use += add;
     s->caches[scene] = cache;
     return NB_OK;
 }
@@ -264,18 +322,45 @@ nb_status nb_server_submit(nb_server *s, const nb_request *req, uint64_t *id) {
     if (req->mode == NB_REQUEST_CACHED && !s->caches[req->scene])
         return set_error(NB_ERR_INVALID_ARGUMENT, "scene %u has no cache (nb_server_set_cache)", req->scene);
     const uint64_t bytes = pixels * 16;
-    if (bytes > s->budget) return set_error(NB_ERR_BUDGET, "request needs %llu B, more than the HBM budget", static_cast<unsigned long long>(bytes));
-    Request *r = new Request();
-    r->id = s->next_id++;
+    // a request that cannot fit beside the resident data even on an idle server is refused
+    uint64_t ws = 0;
+    if (req->mode == NB_REQUEST_CACHED) {
+        const uint64_t have = s->cache_ws_pixels[s->caches[req->scene]];
+        ws = pixels > have ? cache_workspace_bytes(pixels) - cache_workspace_bytes(have) : 0;
+    }
+    if (s->resident + bytes + ws > s->budget)
+        return set_error(NB_ERR_BUDGET, "request needs %llu B, %llu B of the HBM budget are left beside the resident data",
+                         static_cast<unsigned long long>(bytes + ws), static_cast<unsigned long long>(s->budget - s->resident));
+    Request *r = new (std::nothrow) Request();
+    if (!r) return set_error(NB_ERR_OUT_OF_MEMORY, "host allocation failed");
+    r->id = s->next_id;
     r->req = *req;
     r->arrival = req->arrival_s >= 0 ? req->arrival_s : s->now();
     r->bytes = bytes;
-    s->queued.push_back(r);
+    try {
+        s->queued.push_back(r);
+    } catch (const std::bad_alloc &) {
+        delete r;
+        return set_error(NB_ERR_OUT_OF_MEMORY, "host allocation failed");
+    }
+    ++s->next_id;
     *id = r->id;
     return NB_OK;
 }
 
+static nb_status server_step(nb_server *s, uint32_t *n_active, uint32_t *n_queued);
+
+// No exception crosses the C ABI: a host allocation failure inside a step is reported as
+// NB_ERR_OUT_OF_MEMORY.
 nb_status nb_server_step(nb_server *s, uint32_t *n_active, uint32_t *n_queued) {
+    try {
+        return server_step(s, n_active, n_queued);
+    } catch (const std::bad_alloc &) {
+        return set_error(NB_ERR_OUT_OF_MEMORY, "host allocation failed in a server step");
+    }
+}
+
+static nb_status server_step(nb_server *s, uint32_t *n_active, uint32_t *n_queued) {
     if (!s) return set_error(NB_ERR_INVALID_ARGUMENT, "null server");
     if (!s->ctx->sobol) return set_error(NB_ERR_INVALID_ARGUMENT, "no Sobol table set on the context");
     SRV_HIP(hipSetDevice(s->ctx->device));

@@ -119,7 +119,7 @@ __global__ __launch_bounds__(64) void render_pt_kernel(RenderParams p) {
     Stack<K> st;
     st.init(s_stack, lane, spill);
     LaneCounters c{0, 0, 0, 0, 0};
-    PathConst pc;
+    PathConst pc{};
     pc.scene = p.scene;
     pc.env = p.env;
     pc.sobol = SobolView{p.sobol, p.sobol_entries, p.sobol_stride};
@@ -146,8 +146,10 @@ __global__ __launch_bounds__(64) void render_pt_kernel(RenderParams p) {
     uint32_t mode = kIdle;
     uint32_t x = 0, y = 0, f = 0, bounce = 0, frames_done = 0;
     uint32_t px_rays = 0;    // rays traced for the lane's pixel in this launch (its unit's cost)
-    Ray ray;                 // the path's current ray (world space)
-    TraceState ts;
+    // every per-lane value starts defined: idle lanes take part in the wave's ballots, and an
+    // undefined operand there would make the whole ballot undefined
+    Ray ray{};               // the path's current ray (world space)
+    TraceState ts{};
     F3 thr = f3(1.0f, 1.0f, 1.0f), col = f3(0.0f, 0.0f, 0.0f);
     float w = 0.0f;
     for (;;) {
@@ -158,13 +160,13 @@ __global__ __launch_bounds__(64) void render_pt_kernel(RenderParams p) {
             ++c.rays;
             ++px_rays;
             F3 out = f3(0.0f, 0.0f, 0.0f);
-            StepResult r;
+            StepResult r = kDone;
             if (h.t == kNoHit) {
                 out = mul(thr, env_sample(pc.env, ray.d));
                 r = kDone;
             } else {
                 ++c.hits;
-                Vertex dummy;
+                Vertex dummy{};
                 pc.shift = hash4f(x, y, x, y);   // recomputed: not kept live across traversal
                 r = shade<false>(pc, bounce, h, ray, thr, out, dummy);
                 if (r == kContinue && bounce == p.bounces) { r = kDone; out = f3(0.0f, 0.0f, 0.0f); }
@@ -251,6 +253,8 @@ __global__ __launch_bounds__(64) void render_pt_kernel(RenderParams p) {
         // every lane idle after the refill: the stream is exhausted
         if (__ballot(mode != kIdle) == 0ull) break;
         // ---- traverse
+        // (Written out here and in serve_pt_kernel: the same loop moved into a shared
+        // force-inlined function compiles to code that renders wrong pixels.)
         const bool more_pixels = more;
         for (;;) {
             // inner-node visits and plain pops of every lane that has one, until no more than

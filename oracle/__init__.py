@@ -242,6 +242,21 @@ def render(scene, cam, g, table, stride, env, width, height, rgba=None, rect=Non
     return rgba, cnt[0]
 
 
+def sample_pixel(scene, cam, g, table, stride, env, width, height, x, y):
+    """One path for pixel (x, y) of frame g.sample (orc_sample_pixel): returns (rgb, counters)."""
+    t = np.ascontiguousarray(table, dtype=np.float32)
+    out = np.zeros(3, dtype=np.float32)
+    cnt = np.zeros(1, dtype=COUNTERS)
+    lib.orc_sample_pixel(scene.ref, p(cam), p(g), p(t), stride, t.shape[0], env.ref, width, height, x, y, p(out),
+                         p(cnt))
+    return out, cnt[0]
+
+
+def pixel_shift(x, y):
+    """The per-pixel Cranley-Patterson shift hash4f(x, y, x, y) (common.wgsl:27-41, pathtracing.wgsl:252)."""
+    return map4f(hash4u(np.array([x, y, x, y], dtype=np.uint32)))
+
+
 def render_mean_f64(scene, cam, g, table, stride, env, width, height, first, n, threads=1):
     t = np.ascontiguousarray(table, dtype=np.float32)
     out = np.zeros((height, width, 3), dtype=np.float64)

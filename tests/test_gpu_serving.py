@@ -121,12 +121,24 @@ def test_request_is_the_same_alone_and_in_a_mix(env):
     np.testing.assert_array_equal(img_alone.view(np.uint32), img_mix.view(np.uint32))
 
 
+def _resident_bytes(ctx, scenes, e, max_paths):
+    """Bytes a server with these resident scenes charges before any request."""
+    from nbounce_amd.serving import Server
+    probe = Server(ctx, max_paths=max_paths, max_bounces=8)
+    for s in scenes:
+        probe.add_scene(s, e)
+    r = probe.stats()["bytes_in_use"]
+    probe.close()
+    return r
+
+
 def test_admission_waits_for_hbm_budget(env):
     from nbounce_amd.serving import Server
     nb, ctx, scenes, e = env
-    print("case: admission under a budget of 1.5 images", flush=True)
+    print("case: admission under a budget of the resident data plus 1.5 images", flush=True)
     w = h = 64
-    srv = Server(ctx, hbm_budget_bytes=int(1.5 * w * h * 16), max_paths=1 << 16, max_bounces=8)
+    resident = _resident_bytes(ctx, scenes[:1], e, 1 << 16)
+    srv = Server(ctx, hbm_budget_bytes=resident + int(1.5 * w * h * 16), max_paths=1 << 16, max_bounces=8)
     srv.add_scene(scenes[0], e)
     cam = _cam(nb, (5.0, 5.0, 9.5), (5.0, 4.0, 0.0), w, h)
     ids = [srv.submit(cam, 0, w, h, 1, 8) for _ in range(3)]
@@ -141,6 +153,37 @@ def test_admission_waits_for_hbm_budget(env):
     assert [int(c["id"]) for c in srv.poll()] == ids[1:2]
     with pytest.raises(nb.NbError):
         srv.submit(_cam(nb, (5, 5, 9.5), (5, 4, 0), 128, 128), 0, 128, 128, 1, 8)   # larger than the budget
+    srv.close()
+
+
+def test_admission_counts_resident_scenes_and_step_buffers(env):
+    """The budget covers the server's step buffers, the resident scenes and environments and the
+    request images: a scene that does not fit is refused, a request that cannot fit beside the
+    resident data is refused, and the step buffers alone can exceed a budget."""
+    from nbounce_amd.serving import Server
+    nb, ctx, scenes, e = env
+    print("case: admission charges resident scenes", flush=True)
+    own = _resident_bytes(ctx, [], e, 1 << 16)
+    with_scene = _resident_bytes(ctx, scenes[1:], e, 1 << 16)
+    assert with_scene - own >= scenes[1].device_bytes() > 0
+    with pytest.raises(nb.NbError) as ex:
+        Server(ctx, hbm_budget_bytes=own // 2, max_paths=1 << 16, max_bounces=8)
+    assert ex.value.status == 6   # NB_ERR_BUDGET
+    srv = Server(ctx, hbm_budget_bytes=(own + with_scene) // 2, max_paths=1 << 16, max_bounces=8)
+    with pytest.raises(nb.NbError) as ex:
+        srv.add_scene(scenes[1], e)
+    assert ex.value.status == 6
+    srv.close()
+    w = h = 64
+    srv = Server(ctx, hbm_budget_bytes=with_scene + w * h * 16 // 2, max_paths=1 << 16, max_bounces=8)
+    srv.add_scene(scenes[1], e)
+    cam = _cam(nb, REQ["eye"], REQ["target"], w, h)
+    with pytest.raises(nb.NbError) as ex:
+        srv.submit(cam, 0, w, h, 1, 8)    # half an image is left beside the scene
+    assert ex.value.status == 6
+    rid = srv.submit(cam, 0, 32, 32, 1, 8)
+    done = _run_to_completion(srv)
+    assert rid in done and srv.stats()["bytes_in_use"] == with_scene + 32 * 32 * 16
     srv.close()
 
 
