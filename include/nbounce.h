@@ -229,6 +229,15 @@ nb_status nb_counters_enable(nb_context *ctx, uint32_t on);
  * {instance, i0, i1, i2, float bits of dist}, instance = 0xFFFFFFFF on a miss. */
 nb_status nb_primary_hits(nb_context *ctx, const nb_scene *scene, const nb_camera *camera,
                           uint32_t width, uint32_t height, uint32_t sample, uint32_t *out);
+/* nb_primary_hits without any device allocation.  No counterpart in the reference: it is for
+ * callers that cannot allocate device memory (allocation-free interactive picking loops, code
+ * that must not call hipMalloc).  The kernel writes into a fixed library-owned scratch array on
+ * the device (40 MiB per GPU, held until the process ends), so width * height is at most
+ * 2097152 pixels (larger sizes fail with NB_ERR_INVALID_ARGUMENT).  Every call on one GPU uses
+ * that GPU's one scratch array: calls that overlap on one GPU (from several threads or
+ * contexts) race and must be serialized by the caller.  Same output as nb_primary_hits. */
+nb_status nb_primary_hits_noalloc(nb_context *ctx, const nb_scene *scene, const nb_camera *camera,
+                                  uint32_t width, uint32_t height, uint32_t sample, uint32_t *out);
 
 /* ---- serving: continuous batching over resident scenes -------------------------------------
  * No counterpart in the reference (its frame loop, app.rs:177-219, renders one scene for one

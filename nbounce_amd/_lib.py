@@ -61,6 +61,7 @@ SIGNATURES = {
     "nb_counters_reset": [_p],
     "nb_counters_enable": [_p, _u32],
     "nb_primary_hits": [_p, _p, _p, _u32, _u32, _u32, _p],
+    "nb_primary_hits_noalloc": [_p, _p, _p, _u32, _u32, _u32, _p],
     "nb_server_create": [_p, _c.c_uint64, _u32, _u32, _c.POINTER(_p)],
     "nb_server_destroy": [_p],
     "nb_server_add_scene": [_p, _p, _p, _ptr(_u32)],

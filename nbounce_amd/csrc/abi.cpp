@@ -11,6 +11,7 @@
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -40,7 +41,35 @@ nb_status set_error(nb_status st, const char *fmt, ...) {
 
 using namespace nb;
 
+// ---- scratch of nb_primary_hits_noalloc ------------------------------------------------------
+// A file-scope device array, so that nb_primary_hits_noalloc allocates nothing: each GPU holds
+// its own copy for the life of the process, 5 words per pixel for up to kHitsScratchPixels
+// pixels (40 MiB).  In host code `hits_scratch` only names the symbol; its device address on a
+// GPU comes from hipGetSymbolAddress with that GPU current, taken once per device when the first
+// context on it is created and kept in g_hits_scratch.
+constexpr uint32_t kHitsScratchPixels = 1u << 21;
+static __device__ __align__(256) uint32_t hits_scratch[5u * kHitsScratchPixels];
+
 namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_hits_scratch_mutex;
+uint32_t *g_hits_scratch[kMaxDevices] = {};
+
+// Device address of hits_scratch on `device` (the current device), looked up once per device.
+nb_status hits_scratch_address(int device, uint32_t **out) {
+    if (device < 0 || device >= kMaxDevices) return set_error(NB_ERR_UNSUPPORTED, "device %d: at most %d devices", device, kMaxDevices);
+    std::lock_guard<std::mutex> lock(g_hits_scratch_mutex);
+    if (!g_hits_scratch[device]) {
+        void *ptr = nullptr;
+        hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(hits_scratch));
+        if (e != hipSuccess || !ptr)
+            return set_error(NB_ERR_HIP, "scratch symbol address on device %d: %s", device, hipGetErrorString(e));
+        g_hits_scratch[device] = static_cast<uint32_t *>(ptr);
+    }
+    *out = g_hits_scratch[device];
+    return NB_OK;
+}
 
 nb_status use_device(const nb_context *ctx) {
     NB_HIP(hipSetDevice(ctx->device));
@@ -377,6 +406,12 @@ nb_status nb_context_create(int device, nb_context **out) {
         c->closing = true;
         release_context(c);
         return set_error(NB_ERR_HIP, "context creation: %s", hipGetErrorString(e));
+    }
+    // the device address of nb_primary_hits_noalloc's scratch, with this device current
+    if (nb_status st = hits_scratch_address(device, &c->hits_scratch); st != NB_OK) {
+        c->closing = true;
+        release_context(c);
+        return st;
     }
     *out = c;
     return NB_OK;
@@ -747,6 +782,33 @@ nb_status nb_primary_hits(nb_context *ctx, const nb_scene *scene, const nb_camer
     hipFree(d);
     if (st != NB_OK) return st;
     if (e != hipSuccess) return set_error(NB_ERR_HIP, "primary hits: %s", hipGetErrorString(e));
+    return NB_OK;
+}
+
+nb_status nb_primary_hits_noalloc(nb_context *ctx, const nb_scene *scene, const nb_camera *camera, uint32_t width,
+                                  uint32_t height, uint32_t sample, uint32_t *out) {
+    if (!ctx || !scene || !camera || !out) return set_error(NB_ERR_INVALID_ARGUMENT, "null argument");
+    if (scene->ctx != ctx) return set_error(NB_ERR_INVALID_ARGUMENT, "scene belongs to another context");
+    if (!ctx->sobol) return set_error(NB_ERR_INVALID_ARGUMENT, "no Sobol table set (nb_sobol_set)");
+    if (width == 0 || height == 0 || width % 8 || height % 8)
+        return set_error(NB_ERR_INVALID_ARGUMENT, "sides must be nonzero multiples of 8");
+    if (static_cast<uint64_t>(width) * height > kHitsScratchPixels)
+        return set_error(NB_ERR_INVALID_ARGUMENT, "%ux%u exceeds the %u-pixel scratch of nb_primary_hits_noalloc",
+                         width, height, kHitsScratchPixels);
+    NB_TRY(use_device(ctx));
+    // every pixel's five words are written by the kernel, so the scratch needs no clearing
+    nb_target tmp;
+    tmp.ctx = ctx;
+    tmp.width = width;
+    tmp.height = height;
+    nb_globals g{sample, 0.0f, 0u, 0.0f};
+    RenderParams p = make_params(ctx, scene, &tmp, camera, &g, nullptr);
+    p.hits_out = ctx->hits_scratch;
+    p.counters = nullptr;
+    NB_TRY(launch_primary_hits(p, ctx->stream));
+    NB_HIP(hipMemcpyAsync(out, ctx->hits_scratch, static_cast<size_t>(width) * height * 5 * 4, hipMemcpyDeviceToHost,
+                          ctx->stream));
+    NB_HIP(hipStreamSynchronize(ctx->stream));
     return NB_OK;
 }
 

@@ -180,6 +180,7 @@ struct nb_context {
     unsigned long long *counters = nullptr;   // kCounterSlots x kCounterWords
     bool counting = true;                     // nb_counters_enable
     uint32_t *work = nullptr;       // tile counters of the persistent render kernel
+    uint32_t *hits_scratch = nullptr;   // device address of nb_primary_hits_noalloc's scratch (not owned)
     // Scenes, targets and environments keep their context alive: nb_context_destroy only marks
     // it, and the last child's destroy releases it.
     uint32_t children = 0;
