@@ -4,6 +4,7 @@ budget; requests that arrive while others run join the running steps."""
 import numpy as np
 import pytest
 
+import oracle
 from tests import _workloads as W
 
 pytestmark = pytest.mark.gpu
@@ -24,9 +25,35 @@ def env(nb_mod=None):
     sc = nb.Scene(w["vertices"], w["indices_in"].copy(), w["primitives"])
     b, tl, inst = nb.build_scene(sc, share_blas=True)
     sb_large = nb.SceneBuffers(ctx, b, tl, inst, sc.vertices, sc.indices)
-    e = nb.EnvMap(ctx, scenegen.sky_cube(32))
+    faces = scenegen.sky_cube(32)
+    e = nb.EnvMap(ctx, faces)
+    # the reference-layout arrays of both scenes, for the CPU oracle
+    _ORACLE.update(scenes=[oracle.Scene(small["blas"], small["tlas"], small["instances"], small["vertices"],
+                                        small["indices"]),
+                           oracle.Scene(b, tl, inst, sc.vertices, sc.indices)],
+                   env=oracle.Env(faces), table=t)
     yield nb, ctx, [sb_small, sb_large], e
     ctx.close()
+
+
+_ORACLE = {}
+
+
+def _oracle_image(cam, scene, w, h, spp, bounces, cf=4.0):
+    """The CPU oracle's progressive image of samples 1..spp (weights 1/k), pathtracing.wgsl:243-260."""
+    img = np.zeros((h, w, 4), dtype=np.float32)
+    for k in range(1, spp + 1):
+        g = oracle.globals_(k, np.float32(1.0) / np.float32(k), bounces, cf)
+        img, _ = oracle.render(_ORACLE["scenes"][scene], cam, g, _ORACLE["table"], 17, _ORACLE["env"], w, h, rgba=img,
+                               threads=16)
+    return img
+
+
+def _matches_oracle(tag, got, ref, tol=1e-3):
+    err = W.rel_err(got, ref)
+    same = (got.view(np.uint32) == ref.view(np.uint32)).all(axis=-1).mean()
+    print(f"{tag}: bit-identical pixels {same:.5f}, max rel err {err.max():.3e}", flush=True)
+    assert np.isfinite(got).all() and err.max() <= tol
 
 
 def _cam(nb, eye, target, w, h):
@@ -114,11 +141,21 @@ def test_request_is_the_same_alone_and_in_a_mix(env):
     assert mix.stats()["steps"] > 2
     # every request of the mix (both scenes share steps, so waves move between scenes) equals the
     # plain renderer's image of it
+    img_mixed = {}
     for oid, c_k, sc, w, h, spp, bounces in others:
-        np.testing.assert_array_equal(mix.read(oid, w, h).view(np.uint32),
+        img_mixed[oid] = mix.read(oid, w, h)
+        np.testing.assert_array_equal(img_mixed[oid].view(np.uint32),
                                       _plain(nb, ctx, scenes[sc], e, c_k, w, h, spp, bounces).view(np.uint32))
     mix.close()
     np.testing.assert_array_equal(img_alone.view(np.uint32), img_mix.view(np.uint32))
+    # anchored to the CPU oracle: the request alone (= in the mix) and two of the mixed requests,
+    # one per scene, against the oracle's rendering of the same samples
+    _matches_oracle("request alone / in the mix vs oracle", img_alone,
+                    _oracle_image(cam, REQ["scene"], REQ["w"], REQ["h"], REQ["spp"], REQ["bounces"]))
+    for sc_want in (0, 1):
+        oid, c_k, sc, w, h, spp, bounces = next(o for o in others if o[2] == sc_want)
+        _matches_oracle(f"mixed request on scene {sc} ({w}x{h}, {spp} spp, {bounces} bounces) vs oracle",
+                        img_mixed[oid], _oracle_image(c_k, sc, w, h, spp, bounces))
 
 
 def _resident_bytes(ctx, scenes, e, max_paths):
