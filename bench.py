@@ -266,16 +266,21 @@ def serve_cache_bench(args):
     print(json.dumps(out), flush=True)
 
 
-def cache_bench(args):
-    """Learned radiance cache on the spheres-in-box scene at 1920x1080, 8 bounces: 240 frames of
-    one sample while the camera orbits, each frame training the cache on `--train-paths` full
-    paths; then, at the orbit's final pose, relative MSE (every 8th pixel, against the committed
-    float64 4096-spp CPU-oracle reference) and device time of the plain tracer at 1..128 spp and
-    of the cache-assisted tracer (trained network, no further training) at 1..32 spp."""
+def cache_equal_error(train_paths=1 << 16, lr=1e-3, max_batch=None, frames=None, seed=0):
+    """Learned radiance cache on the spheres-in-box scene at 1920x1080, 8 bounces: `frames` (240)
+    frames of one sample while the camera orbits, each frame training the cache on `train_paths`
+    full paths (up to three records each; max_batch records per frame, by default all of them);
+    then, at the orbit's final pose, relative MSE (every 8th pixel, against the committed float64
+    4096-spp CPU-oracle reference) and device time of the plain tracer at 1..128 spp and of the
+    cache-assisted tracer (trained network, no further training) at 1..32 spp.  Returns the
+    equal-error frame-time ratios and the measurements behind them."""
     import nbounce_amd as nb
     from nbounce_amd import scenegen
     from nbounce_amd.cache import RadianceCache, rel_mse
     P = scenegen.CACHE
+    frames = P["frames"] if frames is None else frames
+    if max_batch is None:
+        max_batch = min(3 * train_paths, 1 << 20)
     ref = np.load(os.path.join(ROOT, "tests", "golden", "cache_reference_spheres_1080p.npz"), allow_pickle=False)
     xy, ref_mean = ref["xy"], ref["mean"].astype(np.float64)
     v, i, pr = scenegen.spheres_in_box(0)
@@ -297,13 +302,13 @@ def cache_bench(args):
         img = pt.output_texture()
         return rel_mse(img[xy[:, 1], xy[:, 0]], ref_mean)
 
-    cache = RadianceCache(ctx, [0, 0, 0], [10, 10, 10], seed=0, max_batch=1 << 16)
+    cache = RadianceCache(ctx, [0, 0, 0], [10, 10, 10], seed=seed, max_batch=max_batch)
     ctx.enable_render_timing()
     train_ms, losses = [], []
     plain_orbit_ms = []
-    for k in range(P["frames"]):
+    for k in range(frames):
         pt = tracer(k)
-        losses.append(cache.render(pt, train_paths=args.train_paths, lr=args.lr))
+        losses.append(cache.render(pt, train_paths=train_paths, lr=lr))
         train_ms.append(ctx.last_render_ms())
         if k % 20 == 0:
             pq = tracer(k)
@@ -333,11 +338,10 @@ def cache_bench(args):
     ms_per_spp = plain[-1]["ms"] / plain[-1]["spp"]
     t_plain_eq = spp_eq * ms_per_spp
     t_cache_train = float(np.mean(train_ms[-20:]))
-    out = {
-        "metric": "equal relative-MSE frame time, cache-assisted / plain",
-        "value": round(t1 / t_plain_eq, 4) if np.isfinite(t_plain_eq) else None,
-        "unit": "ratio (lower is better)",
-        "higher_is_better": False,
+    cache.close()
+    ctx.close()
+    return {
+        "ratio": round(t1 / t_plain_eq, 4) if np.isfinite(t_plain_eq) else None,
         "with_training_ratio": round(t_cache_train / t_plain_eq, 4) if np.isfinite(t_plain_eq) else None,
         "cache_1spp": {"rel_mse": e1, "ms": t1},
         "plain_spp_at_equal_error": round(spp_eq, 3),
@@ -347,11 +351,24 @@ def cache_bench(args):
         "loss_first_last": [round(losses[0], 4), round(float(np.mean(losses[-10:])), 4)],
         "plain": plain,
         "cached": cached,
-        "n_gpus": 1,
-        "config": {"workload": "spheres-in-box scene, 1920x1080, 8 bounces, 240-frame orbit, 1 spp per frame",
-                   "train_paths_per_frame": args.train_paths, "lr": args.lr,
+        "config": {"workload": "spheres-in-box scene, 1920x1080, 8 bounces, %d-frame orbit, 1 spp per frame" % frames,
+                   "train_paths_per_frame": train_paths, "lr": lr, "max_records_per_frame": max_batch,
                    "reference": "CPU oracle float64 mean of 4096 spp at every 8th pixel"},
     }
+
+
+def cache_bench(args):
+    """Equal relative-MSE frame time of the cache-assisted renderer over the plain one
+    (cache_equal_error) as one JSON line."""
+    r = cache_equal_error(args.train_paths, args.lr, args.max_batch or None)
+    out = {
+        "metric": "equal relative-MSE frame time, cache-assisted / plain",
+        "value": r.pop("ratio"),
+        "unit": "ratio (lower is better)",
+        "higher_is_better": False,
+        "n_gpus": 1,
+    }
+    out.update(r)
     print(json.dumps(out), flush=True)
 
 
@@ -370,8 +387,10 @@ def main():
     ap.add_argument("--cache-warmup", type=int, default=40, help="serve-cache: training frames per scene cache")
     ap.add_argument("--serve-train-paths", type=int, default=2048,
                     help="serve-cache: training paths per sample of a cache-assisted request")
-    ap.add_argument("--train-paths", type=int, default=21845)
-    ap.add_argument("--lr", type=float, default=2e-3)
+    ap.add_argument("--train-paths", type=int, default=1 << 16, help="cache: training paths per frame")
+    ap.add_argument("--lr", type=float, default=1e-3, help="cache: Adam learning rate")
+    ap.add_argument("--max-batch", type=int, default=0,
+                    help="cache: training records per frame (0: all, three per training path)")
     ap.add_argument("--qps", type=float, default=20.0)
     ap.add_argument("--requests", type=int, default=200)
     ap.add_argument("--max-paths", type=int, default=1 << 23)

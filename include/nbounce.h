@@ -339,6 +339,17 @@ nb_status nb_cache_train(nb_cache *cache, const float *raw, const float *target,
 nb_status nb_render_cached(nb_context *ctx, const nb_scene *scene, nb_target *target, const nb_camera *camera,
                            const nb_globals *globals, const nb_envmap *env, nb_cache *cache, uint32_t train_paths,
                            float lr, float *loss);
+/* The training records nb_render_cached would train on for frame globals->sample of a width x
+ * height view (for inspection and tests; the cache's parameters are not changed).  Training path
+ * i of n traces pixel (i * max(floor(W*H / n), 1) + (sample * 2654435761 mod W*H)) mod W*H with Sobol
+ * row 1 + (sample + 511) mod 1023; each of its first three non-emissive shading points gives a
+ * record: 16 raw floats (as for nb_cache_infer) and the target radiance = the path's radiance /
+ * the throughput that arrived there.  Records come in no particular order; *n is the number
+ * traced (at most the cache's max_batch), of which min(*n, cap) are copied. */
+nb_status nb_cache_training_records(nb_context *ctx, const nb_scene *scene, uint32_t width, uint32_t height,
+                                    const nb_camera *camera, const nb_globals *globals, const nb_envmap *env,
+                                    nb_cache *cache, uint32_t train_paths, float *raw, float *target, uint32_t cap,
+                                    uint32_t *n);
 
 #ifdef __cplusplus
 }

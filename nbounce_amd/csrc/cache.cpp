@@ -354,6 +354,14 @@ nb_status nb_cache_train(nb_cache *c, const float *raw, const float *target, uin
 
 namespace nb {
 
+// Where a frame's training paths start: path i of n traces pixel (i * (W*H / n) + offset) mod W*H
+// with Sobol row train_sample(sample), a row of the 1024-row table away from the frame's own
+// (so the training paths do not repeat the frame's samples).
+static uint32_t train_offset(uint32_t sample, size_t npix) {
+    return static_cast<uint32_t>((static_cast<uint64_t>(sample) * 2654435761ull) % npix);
+}
+static uint32_t train_sample(uint32_t sample) { return 1u + (sample + 511u) % 1023u; }
+
 // One cache-assisted frame into a float4 target of width x height pixels on the context's stream
 // (nb_render_cached's work; also used by the server for cached requests).
 nb_status cache_render_frame(nb_context *ctx, const DevScene &scene, const DevEnv &env, float4 *target, uint32_t width,
@@ -391,8 +399,8 @@ nb_status cache_render_frame(nb_context *ctx, const DevScene &scene, const DevEn
     f.q_thr = c->q_thr;
     f.q_count = c->counts;
     f.train_paths = train_paths;
-    f.train_offset = static_cast<uint32_t>((static_cast<uint64_t>(g.sample) * 2654435761ull) % npix);
-    f.train_sample = 1u + (g.sample + 511u) % 1023u;
+    f.train_offset = train_offset(g.sample, npix);
+    f.train_sample = train_sample(g.sample);
     f.t_capacity = c->max_batch;
     f.t_raw = c->raw;
     f.t_target = c->target;
@@ -445,6 +453,52 @@ uint64_t cache_device_bytes(const nb_cache *c) {
 }  // namespace nb
 
 extern "C" {
+
+nb_status nb_cache_training_records(nb_context *ctx, const nb_scene *scene, uint32_t width, uint32_t height,
+                                    const nb_camera *camera, const nb_globals *g, const nb_envmap *env, nb_cache *c,
+                                    uint32_t train_paths, float *raw, float *target, uint32_t cap, uint32_t *n) {
+    if (!ctx || !scene || !camera || !g || !env || !c || !n || (cap && (!raw || !target)))
+        return set_error(NB_ERR_INVALID_ARGUMENT, "null argument");
+    if (scene->ctx != ctx || env->ctx != ctx || c->ctx != ctx)
+        return set_error(NB_ERR_INVALID_ARGUMENT, "all objects must belong to the context");
+    if (!ctx->sobol) return set_error(NB_ERR_INVALID_ARGUMENT, "no Sobol table set (nb_sobol_set)");
+    if (g->sample == 0) return set_error(NB_ERR_INVALID_ARGUMENT, "samples are 1-based");
+    if (!width || !height || width % 8 || height % 8 || train_paths == 0)
+        return set_error(NB_ERR_INVALID_ARGUMENT, "need a nonzero multiple-of-8 size and training paths");
+    C_HIP(hipSetDevice(ctx->device));
+    const size_t npix = static_cast<size_t>(width) * height;
+    CacheFrame f{};
+    f.scene = scene->dev;
+    f.env = DevEnv{env->texels, env->n};
+    f.sobol = ctx->sobol;
+    f.sobol_entries = ctx->sobol_entries;
+    f.sobol_stride = ctx->sobol_stride;
+    std::memcpy(f.c2w, camera->clip_to_world, sizeof(f.c2w));
+    f.width = width;
+    f.height = height;
+    f.sample = g->sample;
+    f.bounces = g->bounces;
+    f.contribution_factor = g->contribution_factor;
+    f.train_paths = train_paths;
+    f.train_offset = train_offset(g->sample, npix);
+    f.train_sample = train_sample(g->sample);
+    f.t_capacity = c->max_batch;
+    f.t_raw = c->raw;
+    f.t_target = c->target;
+    f.t_count = c->counts + 1;
+    C_TRY(cache_launch_train_paths(f, ctx->stream));
+    uint32_t count = 0;
+    C_HIP(hipMemcpyAsync(&count, c->counts + 1, 4, hipMemcpyDeviceToHost, ctx->stream));
+    C_HIP(hipStreamSynchronize(ctx->stream));
+    if (count > c->max_batch) count = c->max_batch;
+    const uint32_t k = count < cap ? count : cap;
+    if (k) {
+        C_HIP(hipMemcpy(raw, c->raw, static_cast<size_t>(k) * kMlpRaw * 4, hipMemcpyDeviceToHost));
+        C_HIP(hipMemcpy(target, c->target, static_cast<size_t>(k) * 12, hipMemcpyDeviceToHost));
+    }
+    *n = count;
+    return NB_OK;
+}
 
 nb_status nb_render_cached(nb_context *ctx, const nb_scene *scene, nb_target *target, const nb_camera *camera,
                            const nb_globals *g, const nb_envmap *env, nb_cache *c, uint32_t train_paths, float lr,

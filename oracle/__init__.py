@@ -63,6 +63,8 @@ def _load():
         [ctypes.c_uint32] * 2 + [ctypes.c_void_p] + [ctypes.c_uint32] * 3 + [ctypes.c_void_p, ctypes.c_int]
     lib.orc_sample_pixel.argtypes = [ctypes.c_void_p] * 4 + [ctypes.c_uint32] * 2 + [ctypes.c_void_p] + \
         [ctypes.c_uint32] * 4 + [ctypes.c_void_p, ctypes.c_void_p]
+    lib.orc_train_path.argtypes = [ctypes.c_void_p] * 4 + [ctypes.c_uint32] * 2 + [ctypes.c_void_p] + \
+        [ctypes.c_uint32] * 4 + [ctypes.c_void_p] * 4
     lib.orc_primary_hits.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32,
                                      ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_void_p]
     lib.orc_build_scene.argtypes = [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_void_p, ctypes.c_uint32,
@@ -250,6 +252,19 @@ def sample_pixel(scene, cam, g, table, stride, env, width, height, x, y):
     lib.orc_sample_pixel(scene.ref, p(cam), p(g), p(t), stride, t.shape[0], env.ref, width, height, x, y, p(out),
                          p(cnt))
     return out, cnt[0]
+
+
+def train_path(scene, cam, g, table, stride, env, width, height, x, y):
+    """One path of pixel (x, y) with its training records (orc_train_path): returns
+    (raw (n, 16), target (n, 3), radiance (3,))."""
+    t = np.ascontiguousarray(table, dtype=np.float32)
+    raw = np.zeros((3, 16), dtype=np.float32)
+    target = np.zeros((3, 3), dtype=np.float32)
+    rad = np.zeros(3, dtype=np.float32)
+    n = ctypes.c_uint32()
+    lib.orc_train_path(scene.ref, p(cam), p(g), p(t), stride, t.shape[0], env.ref, width, height, x, y, p(raw),
+                       p(target), ctypes.byref(n), p(rad))
+    return raw[:n.value], target[:n.value], rad
 
 
 def pixel_shift(x, y):

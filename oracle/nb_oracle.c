@@ -998,9 +998,14 @@ static void sobol_fetch(const float *table, uint32_t entries, uint32_t entry, co
 }
 
 /* sample_rendering_eq, pathtracing.wgsl:161-241 */
-static v3 sample_rendering_eq(const orc_scene *sc, const orc_globals *g, const float *table, uint32_t stride,
-                              uint32_t table_entries, const orc_env *env, const float shift[4], ray_t ray,
-                              orc_counters *cnt) {
+/* Shading points of a path as the learned cache's training records see them (no counterpart in
+ * the reference): up to max_rec non-emissive hits, each as 16 floats (position, outgoing
+ * direction, normal, roughness, albedo, metallic, 0, 0) with the throughput that arrived there. */
+typedef struct { float *raw; v3 *thr; uint32_t n, max; } path_rec;
+
+static v3 sample_rendering_eq_rec(const orc_scene *sc, const orc_globals *g, const float *table, uint32_t stride,
+                                  uint32_t table_entries, const orc_env *env, const float shift[4], ray_t ray,
+                                  orc_counters *cnt, path_rec *rec) {
     v3 thr = mk3(1.0f, 1.0f, 1.0f);
     for (uint32_t bounce = 0; bounce <= g->bounces; ++bounce) {
         hit_info hit = intersect_scene(sc, &ray, cnt);
@@ -1013,6 +1018,18 @@ static v3 sample_rendering_eq(const orc_scene *sc, const orc_globals *g, const f
         float alpha = hit.roughness * hit.roughness;
         float alpha2 = alpha * alpha;
         v3 n = normalize3(hit.normal);
+        if (rec && rec->n < rec->max) {
+            v3 wo_ = normalize3(mk3(-ray.d.x, -ray.d.y, -ray.d.z));
+            float *r = rec->raw + 16u * rec->n;
+            r[0] = hit.position.x; r[1] = hit.position.y; r[2] = hit.position.z;
+            r[3] = wo_.x; r[4] = wo_.y; r[5] = wo_.z;
+            r[6] = n.x; r[7] = n.y; r[8] = n.z;
+            r[9] = hit.roughness;
+            r[10] = hit.color.x; r[11] = hit.color.y; r[12] = hit.color.z;
+            r[13] = hit.metallic; r[14] = 0.0f; r[15] = 0.0f;
+            rec->thr[rec->n] = thr;
+            rec->n += 1;
+        }
         float s0[4], s1[4];
         uint32_t e0 = g->sample * stride + 1u + bounce * 2u;
         sobol_fetch(table, table_entries, e0, shift, s0);
@@ -1065,6 +1082,35 @@ static v3 sample_rendering_eq(const orc_scene *sc, const orc_globals *g, const f
         ray.o = hit.position; ray.d = wi; ray.inv = div3(mk3(1.0f, 1.0f, 1.0f), wi);
     }
     return mk3(0.0f, 0.0f, 0.0f);
+}
+
+static v3 sample_rendering_eq(const orc_scene *sc, const orc_globals *g, const float *table, uint32_t stride,
+                              uint32_t table_entries, const orc_env *env, const float shift[4], ray_t ray,
+                              orc_counters *cnt) {
+    return sample_rendering_eq_rec(sc, g, table, stride, table_entries, env, shift, ray, cnt, NULL);
+}
+
+/* One path of pixel (x, y) with its first three shading points as training records: target =
+ * the path's radiance divided by the throughput that arrived at the point (0 where that is 0). */
+void orc_train_path(const orc_scene *sc, const orc_camera *cam, const orc_globals *g, const float *table,
+                    uint32_t stride, uint32_t rows, const orc_env *env, uint32_t w, uint32_t h,
+                    uint32_t x, uint32_t y, float raw[48], float target[9], uint32_t *n_rec, float radiance[3]) {
+    float shift[4];
+    pixel_shift(x, y, shift);
+    float jit[4];
+    sobol_fetch(table, rows * stride, g->sample * stride, shift, jit);
+    ray_t ray = generate_ray(cam, x, y, w, h, jit[0], jit[1]);
+    v3 thr[3];
+    path_rec rec = {raw, thr, 0u, 3u};
+    orc_counters cnt = {0, 0, 0, 0};
+    v3 out = sample_rendering_eq_rec(sc, g, table, stride, rows * stride, env, shift, ray, &cnt, &rec);
+    for (uint32_t k = 0; k < rec.n; ++k) {
+        target[3 * k + 0] = thr[k].x > 0.0f ? out.x / thr[k].x : 0.0f;
+        target[3 * k + 1] = thr[k].y > 0.0f ? out.y / thr[k].y : 0.0f;
+        target[3 * k + 2] = thr[k].z > 0.0f ? out.z / thr[k].z : 0.0f;
+    }
+    *n_rec = rec.n;
+    radiance[0] = out.x; radiance[1] = out.y; radiance[2] = out.z;
 }
 
 void orc_sample_pixel(const orc_scene *sc, const orc_camera *cam, const orc_globals *g, const float *table,

@@ -31,7 +31,17 @@
  *   orc_env_sample            texture.rs:126-141, pathtracing.wgsl:168 (Vulkan cube rules)
  *                                                        constant / linear fields, face centres
  *   orc_render                pathtracing.wgsl:27-272    white-furnace / closed-form single-bounce
- *                                                        expectations, RR unbiasedness
+ *                                                        expectations, RR unbiasedness; the
+ *                                                        multi-bounce loop by hand-derived path
+ *                                                        lengths in closed boxes: roulette length
+ *                                                        term 1-(b/B)^8 (exact scan), per-bounce
+ *                                                        Sobol entries i*17+1+2b / +2+2b (marker
+ *                                                        tables, dependency probes), mean path
+ *                                                        length = sum_b prod_k<b (1-(k/B)^8)
+ *   orc_train_path            (no reference: the learned cache's training records) the same
+ *                                                        path as orc_sample_pixel (equal radiance),
+ *                                                        records = its shading points, target x
+ *                                                        throughput = radiance
  */
 #ifndef NB_ORACLE_H
 #define NB_ORACLE_H
@@ -142,6 +152,14 @@ void orc_sample_pixel(const orc_scene *scene, const orc_camera *cam, const orc_g
                       const float *sobol_table, uint32_t sobol_stride, uint32_t sobol_rows,
                       const orc_env *env, uint32_t width, uint32_t height,
                       uint32_t x, uint32_t y, float out[3], orc_counters *counters);
+/* One path for pixel (x, y) with its first three non-emissive shading points as training records
+ * of the learned cache (no counterpart in the reference): raw[16k..] = position, outgoing
+ * direction, normal, roughness, albedo, metallic, 0, 0; target[3k..] = radiance / throughput
+ * arriving there (0 where that throughput is 0). */
+void orc_train_path(const orc_scene *scene, const orc_camera *cam, const orc_globals *g,
+                    const float *sobol_table, uint32_t sobol_stride, uint32_t sobol_rows,
+                    const orc_env *env, uint32_t width, uint32_t height, uint32_t x, uint32_t y,
+                    float raw[48], float target[9], uint32_t *n_rec, float radiance[3]);
 /* Same integrator but sums n_samples samples per pixel (rows first..first+n-1) in float64:
  * out64[p*3+c] = mean radiance. Used as the high-spp reference image. */
 void orc_render_mean_f64(const orc_scene *scene, const orc_camera *cam, const orc_globals *g,

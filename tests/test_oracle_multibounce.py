@@ -230,3 +230,29 @@ def test_mean_path_length_over_the_sobol_table(bounces):
     assert abs(mean / expected_rays(bounces) - 1.0) < 0.005, (mean, expected_rays(bounces))
     for e in (4, 5):
         assert abs(mean / expected_rays(bounces, e) - 1.0) > 0.05
+
+
+def test_training_path_is_the_sample_path():
+    """orc_train_path follows orc_sample_pixel's path (same radiance bits) and records its first
+    three non-emissive shading points; the first arrives with throughput 1, so its target is the
+    radiance itself, and the first record's position lies on the primary ray's closest hit."""
+    from nbounce_amd import scenegen as sg
+    from tests import _workloads as Wk
+    wl = Wk.small()
+    osc = Wk.oracle_scene(wl)
+    env = oracle.Env(sg.sky_cube(16))
+    cam = Wk.camera(sg.SMALL["eye"], sg.SMALL["target"], 1.0)
+    table = Wk.table()
+    n_paths = 0
+    for k in range(64):
+        x, y = (37 * k) % 64, (11 * k) % 64
+        g = oracle.globals_(1 + k % 7, 1.0, 8, 4.0)
+        raw, tgt, rad = oracle.train_path(osc, cam, g, table, STRIDE, env, 64, 64, x, y)
+        out, _ = oracle.sample_pixel(osc, cam, g, table, STRIDE, env, 64, 64, x, y)
+        assert np.array_equal(rad.view(np.uint32), out.view(np.uint32))
+        assert len(raw) <= 3 and np.isfinite(tgt).all()
+        if len(raw):
+            n_paths += 1
+            assert np.array_equal(tgt[0].view(np.uint32), rad.view(np.uint32))
+            assert np.all(raw[:, 14:] == 0) and np.allclose(np.linalg.norm(raw[:, 3:6], axis=1), 1, atol=1e-5)
+    assert n_paths > 32
