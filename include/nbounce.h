@@ -161,6 +161,28 @@ nb_status nb_sobol_set(nb_context *ctx, const float *table, uint32_t rows, uint3
  * texels: 6 faces in Vulkan layer order +X,-X,+Y,-Y,+Z,-Z, each face_size^2 RGBA32F, row-major. */
 nb_status nb_envmap_create_cube(nb_context *ctx, const float *texels, uint32_t face_size, nb_envmap **out);
 nb_status nb_envmap_destroy(nb_envmap *env);
+/* EnvMap::load (src/pathtracing/envmap.rs:12 -> Texture::create_cubemap, src/common/texture.rs:89-141):
+ * a cube DDS file, decoded on the host to RGBA32F (mip 0 of each face; the shader samples LOD 0,
+ * pathtracing.wgsl:168) and uploaded as nb_envmap_create_cube does.  Formats: BC6H_UF16 (the
+ * reference's only one), BC6H_SF16, R16G16B16A16_FLOAT, R32G32B32A32_FLOAT (DX10 header, or the
+ * legacy D3DFMT codes 113 / 116).  The six faces are read face-major with their mip chains
+ * whatever the header's array size (the reference forces it to 6, texture.rs:95).  Unsupported
+ * or truncated files fail with NB_ERR_FORMAT, unreadable ones with NB_ERR_IO. */
+nb_status nb_envmap_load(nb_context *ctx, const char *path, nb_envmap **out);
+/* The decoding alone, into 6 * face_size^2 RGBA32F texels (rgba == NULL: only *face_size). */
+nb_status nb_dds_read_cube(const char *path, float *rgba, uint64_t cap_floats, uint32_t *face_size);
+nb_status nb_dds_read_cube_memory(const void *data, size_t size, float *rgba, uint64_t cap_floats,
+                                  uint32_t *face_size);
+/* Writes a cube DDS (DX10 header, one mip level) from 6 faces of RGBA32F texels: as RGBA32F,
+ * RGBA16F (rounded to nearest), or BC6H_UF16 (one-region blocks, for tests and tools).  No
+ * counterpart in the reference, which only reads. */
+#define NB_DDS_RGBA32F 0u
+#define NB_DDS_RGBA16F 1u
+#define NB_DDS_BC6H_UF16 2u
+nb_status nb_dds_write_cube(const char *path, const float *rgba, uint32_t face_size, uint32_t format);
+/* One BC6H block (16 bytes) decoded to 16 texels x RGB half-float bits, row-major; is_signed
+ * selects BC6H_SF16.  Reserved modes decode to zero. */
+nb_status nb_bc6h_decode_block(const uint8_t *block, uint32_t is_signed, uint16_t *rgb_half);
 
 /* ---- camera (CameraController::calc_camera_data, src/common/camera.rs:121-142) ---------- */
 nb_status nb_camera_look_at(const float eye[3], const float target[3], const float up[3],

@@ -39,6 +39,11 @@ SIGNATURES = {
     "nb_sobol_set": [_p, _p, _u32, _u32, _u32],
     "nb_envmap_create_cube": [_p, _p, _u32, _c.POINTER(_p)],
     "nb_envmap_destroy": [_p],
+    "nb_envmap_load": [_p, _c.c_char_p, _c.POINTER(_p)],
+    "nb_dds_read_cube": [_c.c_char_p, _p, _c.c_uint64, _ptr(_u32)],
+    "nb_dds_read_cube_memory": [_p, _c.c_size_t, _p, _c.c_uint64, _ptr(_u32)],
+    "nb_dds_write_cube": [_c.c_char_p, _p, _u32, _u32],
+    "nb_bc6h_decode_block": [_p, _u32, _p],
     "nb_camera_look_at": [_p, _p, _p, _f32, _f32, _f32, _p],
     "nb_target_create": [_p, _u32, _u32, _c.POINTER(_p)],
     "nb_target_destroy": [_p],

@@ -15,7 +15,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 CSRC = os.path.join(ROOT, "nbounce_amd", "csrc")
 BUILD = os.path.join(ROOT, "_build")
 LIB = os.path.join(ROOT, "nbounce_amd", "libnbounce.so")
-SOURCES = ["trace.hip", "order.hip", "serve.hip", "mlp.hip", "cache_render.hip", "display.hip", "abi.cpp", "server.cpp", "cache.cpp", "scene_build.cpp", "sobol.cpp", "gltf.cpp", "image_io.cpp"]
+SOURCES = ["trace.hip", "order.hip", "serve.hip", "mlp.hip", "cache_render.hip", "display.hip", "abi.cpp", "server.cpp", "cache.cpp", "scene_build.cpp", "sobol.cpp", "gltf.cpp", "image_io.cpp", "dds.cpp"]
 ARCH = os.environ.get("NB_OFFLOAD_ARCH", "gfx950")
 
 

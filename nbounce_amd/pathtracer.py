@@ -5,6 +5,7 @@ src/common/camera.rs; the window / wgpu context is replaced by `Context` (one HI
 explicit output size.
 """
 import ctypes
+import os
 import math
 
 import numpy as np
@@ -197,8 +198,28 @@ class CameraController:
         return cam
 
 
+DDS_RGBA32F, DDS_RGBA16F, DDS_BC6H_UF16 = 0, 1, 2
+
+
+def read_dds_cube(path):
+    """Mip 0 of the six faces of a cube DDS file as (6, n, n, 4) RGBA32F (nb_dds_read_cube)."""
+    n = ctypes.c_uint32()
+    p = os.fsencode(path)
+    check(lib.nb_dds_read_cube(p, None, 0, ctypes.byref(n)))
+    faces = np.zeros((6, n.value, n.value, 4), dtype=np.float32)
+    check(lib.nb_dds_read_cube(p, ptr(faces), faces.size, ctypes.byref(n)))
+    return faces
+
+
+def write_dds_cube(path, faces, fmt=DDS_RGBA32F):
+    """Writes (6, n, n, 4) RGBA32F faces as a cube DDS (RGBA32F, RGBA16F or BC6H_UF16)."""
+    faces = np.ascontiguousarray(faces, dtype=np.float32)
+    check(lib.nb_dds_write_cube(os.fsencode(path), ptr(faces), faces.shape[1], fmt))
+
+
 class EnvMap:
-    """Cube environment map (EnvMap, src/pathtracing/envmap.rs:5-26) from RGBA32F faces."""
+    """Cube environment map (EnvMap, src/pathtracing/envmap.rs:5-26) from RGBA32F faces, or from
+    a cube DDS file with EnvMap.load (envmap.rs:12)."""
 
     def __init__(self, ctx, faces):
         faces = np.ascontiguousarray(faces, dtype=np.float32)
@@ -208,6 +229,17 @@ class EnvMap:
         check(lib.nb_envmap_create_cube(ctx.handle, ptr(faces), faces.shape[1], ctypes.byref(h)))
         self.handle = h
         self.faces = faces
+
+    @classmethod
+    def load(cls, ctx, path):
+        """EnvMap::load (envmap.rs:12): a BC6H (or RGBA16F / RGBA32F) cube DDS, decoded on the host
+        and uploaded (nb_envmap_load); `faces` holds the decoded texels."""
+        self = cls.__new__(cls)
+        h = ctypes.c_void_p()
+        check(lib.nb_envmap_load(ctx.handle, os.fsencode(path), ctypes.byref(h)))
+        self.handle = h
+        self.faces = read_dds_cube(path)
+        return self
 
     def close(self):
         if getattr(self, "handle", None):
