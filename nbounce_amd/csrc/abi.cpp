@@ -5,6 +5,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdarg>
 #include <cstdlib>
@@ -361,6 +362,90 @@ RenderParams make_params(nb_context *ctx, const nb_scene *scene, nb_target *t, c
     return p;
 }
 
+// ---- sample-parallel frames ----------------------------------------------------------------
+// nb_render_frames keeps a pixel's frames in one lane (the mix chain runs in order).  A target of
+// fewer pixels than the GPU holds lanes (4096 resident waves x 64 at 4 waves per SIMD) leaves
+// most of them idle, so such calls render one path per (frame, pixel) with the serving kernels
+// instead: the frames' paths in 16-path units (serve_pt_kernel, one request, one pass), each
+// path's radiance in a result slot, then the ordered accumulation mixes each pixel's samples in
+// frame order with weight 1/sample.  Every sample and the mix chain are the same, so the image
+// is the same bit for bit.  NB_SAMPLE_PARALLEL=0 turns this off, NB_SAMPLE_PARALLEL_PIXELS
+// moves the size limit.
+constexpr uint32_t kSpPathsPerLaunch = 1u << 22;   // result slots: 64 MiB
+
+uint32_t sample_parallel_pixels() {
+    static const uint32_t v = [] {
+        const char *e = std::getenv("NB_SAMPLE_PARALLEL");
+        if (e && std::atoi(e) == 0) return 0u;
+        const char *n = std::getenv("NB_SAMPLE_PARALLEL_PIXELS");
+        return n ? static_cast<uint32_t>(std::strtoul(n, nullptr, 10)) : (1u << 19);
+    }();
+    return v;
+}
+
+struct SpDesc {
+    ServeStep step;
+    ServePass pass;
+    ServeGroup group;
+    ServeRequestDev request;
+    ServeSceneDev scene;
+};
+
+nb_status render_frames_sample_parallel(nb_context *ctx, const nb_scene *scene, nb_target *t, const nb_camera *cam,
+                                        const nb_globals *first, uint32_t n_frames, const nb_envmap *env) {
+    const uint32_t pixels = t->width * t->height;
+    if (!ctx->sp_block) {
+        const size_t desc = (sizeof(SpDesc) + 255) / 256 * 256;
+        const size_t bytes = desc + kWorkStride * sizeof(uint32_t) + static_cast<size_t>(kSpPathsPerLaunch) * 16;
+        NB_HIP(hipMalloc(&ctx->sp_block, bytes));
+        ctx->sp_results = reinterpret_cast<float4 *>(static_cast<char *>(ctx->sp_block) + desc + kWorkStride * sizeof(uint32_t));
+        ctx->sp_max_paths = kSpPathsPerLaunch;
+    }
+    SpDesc *d = static_cast<SpDesc *>(ctx->sp_block);
+    uint32_t *work = reinterpret_cast<uint32_t *>(static_cast<char *>(ctx->sp_block) + (sizeof(SpDesc) + 255) / 256 * 256);
+    const uint32_t per_launch = std::max(1u, ctx->sp_max_paths / pixels);
+    for (uint32_t done = 0; done < n_frames;) {
+        const uint32_t count = std::min(per_launch, n_frames - done);
+        SpDesc h{};
+        h.scene = ServeSceneDev{scene->dev, DevEnv{env->texels, env->n}};
+        std::memcpy(h.request.c2w, cam->clip_to_world, sizeof(h.request.c2w));
+        h.request.target = t->pixels;
+        h.request.width = t->width;
+        h.request.height = t->height;
+        h.request.scene = 0;
+        h.request.bounces = first->bounces;
+        h.request.contribution_factor = first->contribution_factor;
+        h.pass = ServePass{0u, first->sample + done, count, pixels, 0u, 0u, 0u, 0u};
+        h.group = ServeGroup{0u, 0u, count * (pixels / 16u), 0u, 1u, {0u, 0u, 0u}};
+        ServeStep &st = h.step;
+        st.scenes = &d->scene;
+        st.requests = &d->request;
+        st.passes = &d->pass;
+        st.groups = &d->group;
+        st.n_passes = 1;
+        st.n_groups = 1;
+        st.n_units = count * (pixels / 16u);
+        st.n_pixels = pixels;
+        st.sobol = ctx->sobol;
+        st.sobol_entries = ctx->sobol_entries;
+        st.sobol_stride = ctx->sobol_stride;
+        st.results = ctx->sp_results;
+        st.work = work;
+        st.counters = ctx->counting ? ctx->counters : nullptr;
+        st.regen_min_idle = 24;
+        st.inner_quorum = 28;
+        st.grab_units = 4;
+        NB_HIP(hipMemcpyAsync(d, &h, sizeof(SpDesc), hipMemcpyHostToDevice, ctx->stream));
+        NB_TRY(serve_launch_reset(&d->step, ctx->stream));
+        NB_TRY(serve_launch_step(&d->step, ctx->stream));
+        int cus = 0;
+        NB_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ctx->device));
+        NB_TRY(serve_launch_accumulate(&d->step, static_cast<uint32_t>(cus > 0 ? cus : 64) * 8u, ctx->stream));
+        done += count;
+    }
+    return NB_OK;
+}
+
 nb_status timed_frames(nb_context *ctx, const RenderParams &p) {
     // event records between launches cost the stream about 0.5 % at 1080p, so they are made only
     // when the caller asked for per-render times
@@ -423,6 +508,7 @@ static void release_context(nb_context *ctx) {
     hipFree(ctx->sobol);
     hipFree(ctx->counters);
     hipFree(ctx->work);
+    hipFree(ctx->sp_block);
     if (ctx->ev_start) hipEventDestroy(ctx->ev_start);
     if (ctx->ev_stop) hipEventDestroy(ctx->ev_stop);
     if (ctx->ev_region_start) hipEventDestroy(ctx->ev_region_start);
@@ -687,6 +773,15 @@ nb_status nb_render_frames(nb_context *ctx, const nb_scene *scene, nb_target *ta
     if (first->sample == 0 || first->sample > 0xFFFFFFFFu - n_frames)
         return set_error(NB_ERR_INVALID_ARGUMENT, "frames must have 1-based sample numbers");
     NB_TRY(use_device(ctx));
+    const uint32_t pixels = target->width * target->height;
+    if (n_frames >= 2 && pixels <= sample_parallel_pixels() && pixels <= kSpPathsPerLaunch) {
+        if (!ctx->render_timing) return render_frames_sample_parallel(ctx, scene, target, camera, first, n_frames, env);
+        NB_HIP(hipEventRecord(ctx->ev_start, ctx->stream));
+        NB_TRY(render_frames_sample_parallel(ctx, scene, target, camera, first, n_frames, env));
+        NB_HIP(hipEventRecord(ctx->ev_stop, ctx->stream));
+        ctx->timed = true;
+        return NB_OK;
+    }
     RenderParams p = make_params(ctx, scene, target, camera, first, env);
     p.n_frames = n_frames;
     p.weights_from_sample = 1;

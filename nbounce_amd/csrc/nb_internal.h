@@ -181,6 +181,11 @@ struct nb_context {
     bool counting = true;                     // nb_counters_enable
     uint32_t *work = nullptr;       // tile counters of the persistent render kernel
     uint32_t *hits_scratch = nullptr;   // device address of nb_primary_hits_noalloc's scratch (not owned)
+    // Sample-parallel nb_render_frames (small targets): the serving kernels' step description
+    // (one request, one pass, one group) and per-path results, allocated at first use.
+    void *sp_block = nullptr;
+    float4 *sp_results = nullptr;
+    uint32_t sp_max_paths = 0;
     // Scenes, targets and environments keep their context alive: nb_context_destroy only marks
     // it, and the last child's destroy releases it.
     uint32_t children = 0;

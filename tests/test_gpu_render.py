@@ -85,6 +85,35 @@ def test_frames_equal_sequential_dispatches(nb, ctx, small_gpu):
     assert a.sample_count() == b.sample_count() == 4
 
 
+@pytest.mark.parametrize("size,frames", [((256, 256), 16), ((512, 512), 20), ((8, 8), 3)])
+def test_sample_parallel_frames_equal_sequential_dispatches(nb, ctx, small_gpu, size, frames):
+    """Small targets render nb_render_frames one path per (frame, pixel) over the whole GPU (the
+    serving kernels, ordered accumulation): the image and the ray counts equal `frames`
+    sequential dispatches.  512x512 x 20 frames exceeds one launch's result slots (2^22 paths),
+    so it also crosses a launch boundary; frames start at sample 3 on a non-empty target."""
+    w, sb, env = small_gpu
+    from nbounce_amd import scenegen
+    P = scenegen.SMALL
+    print(f"case: {size[0]}x{size[1]}, frames 3..{frames + 2} in one call vs dispatches", flush=True)
+    a, _ = _pathtracer(nb, ctx, sb, env, size, P["eye"], P["target"])
+    b, _ = _pathtracer(nb, ctx, sb, env, size, P["eye"], P["target"])
+    for p in (a, b):
+        p.dispatch()
+        p.dispatch()
+    ctx.reset_counters()
+    for _ in range(frames):
+        a.dispatch()
+    ca = ctx.counters()
+    ctx.reset_counters()
+    b.dispatch_frames(frames)
+    cb = ctx.counters()
+    np.testing.assert_array_equal(a.output_texture().view(np.uint32), b.output_texture().view(np.uint32))
+    assert a.sample_count() == b.sample_count() == frames + 2
+    for k in ("rays", "n_aabb", "n_tri", "n_inst", "samples", "hits"):
+        assert ca[k] == cb[k], k
+    assert cb["samples"] == size[0] * size[1] * frames
+
+
 def test_progressive_frames_match_oracle(nb, ctx, small_gpu):
     w, sb, env = small_gpu
     from nbounce_amd import scenegen
