@@ -102,19 +102,9 @@ def calibration_copy(mib):
     hip.hipFree(b)
 
 
-def serve_bench(args):
-    """Serving workload: 4 resident instanced scenes (seeds 0-3 with 1024..4096 instances),
-    Poisson arrivals at --qps, resolution from {256^2, 512^2, 1024x768, 1920x1080} with
-    p = 0.4/0.3/0.2/0.1, spp uniform over {1, 4, 16, 64}, random cameras, 8 bounces, HBM
-    admission budget 200 GB.  Reports request latency p50/p99 and aggregate Mrays/s."""
-    import nbounce_amd as nb
-    from nbounce_amd import scenegen, serving
-    ctx = nb.Context(0)
-    table = nb.sobol_table(1024, 17, 0)
-    from nbounce_amd._lib import check, lib, ptr
-    check(lib.nb_sobol_set(ctx.handle, ptr(table), 1024, 17, 17))
-    srv = serving.Server(ctx, hbm_budget_bytes=200 << 30, max_paths=args.max_paths, max_bounces=12)
-    env = nb.EnvMap(ctx, scenegen.sky_cube(256))
+def serving_scenes(nb, ctx, srv, env):
+    """The serving workload's 4 resident instanced scenes (seeds 0-3, 1024..4096 instances)."""
+    from nbounce_amd import scenegen
     keep = []
     for seed, n_inst in zip(range(4), (1024, 2048, 3072, 4096)):
         v, i, pr = scenegen.instanced_meshes(seed, n_instances=n_inst)
@@ -123,39 +113,101 @@ def serve_bench(args):
         sb = nb.SceneBuffers(ctx, b, t, inst, sc.vertices, sc.indices)
         srv.add_scene(sb, env)
         keep.append(sb)
-    reqs = serving.mixed_requests(args.requests, args.qps, 4, seed=0)
+    return keep
+
+
+def serve_streams(qps_list, requests, max_paths=1 << 23):
+    """Serving workload: 4 resident instanced scenes (seeds 0-3 with 1024..4096 instances),
+    Poisson arrivals at each QPS in turn (a fresh stream each), resolution from {256^2, 512^2,
+    1024x768, 1920x1080} with p = 0.4/0.3/0.2/0.1, spp uniform over {1, 4, 16, 64}, random
+    cameras, 8 bounces, HBM admission budget 200 GB.  Returns per QPS: request latency p50/p99,
+    aggregate Mrays/s and the GPU's busy fraction."""
+    import nbounce_amd as nb
+    from nbounce_amd import scenegen, serving
+    from nbounce_amd._lib import check, lib, ptr
+    ctx = nb.Context(0)
+    table = nb.sobol_table(1024, 17, 0)
+    check(lib.nb_sobol_set(ctx.handle, ptr(table), 1024, 17, 17))
+    srv = serving.Server(ctx, hbm_budget_bytes=200 << 30, max_paths=max_paths, max_bounces=12)
+    env = nb.EnvMap(ctx, scenegen.sky_cube(256))
+    keep = serving_scenes(nb, ctx, srv, env)
 
     def camera(r):
         return nb.CameraController(world_position=r["eye"], target=r["target"],
                                    aspect_ratio=r["width"] / r["height"]).data()
 
-    # warm-up: two small requests so first-use costs stay out of the timed run
-    serving.run_open_loop(srv, [dict(r, arrival=0.0) for r in reqs[:2]], camera)
-    s0 = srv.stats()
-    lat, wall, _ = serving.run_open_loop(srv, reqs, camera)
-    s1 = srv.stats()
-    rays = s1["rays"] - s0["rays"]
-    out = {
-        "metric": "serving request latency p50/p99 and aggregate Mrays/s",
-        "value": round(float(np.percentile(lat, 99)) * 1e3, 2),
-        "unit": "ms (p99 latency)",
-        "higher_is_better": False,
-        "p50_ms": round(float(np.percentile(lat, 50)) * 1e3, 2),
-        "p99_ms": round(float(np.percentile(lat, 99)) * 1e3, 2),
-        "mean_ms": round(float(lat.mean()) * 1e3, 2),
-        "aggregate_mrays_s": round(rays / wall / 1e6, 2),
-        "gpu_busy_frac": round((s1["gpu_busy_s"] - s0["gpu_busy_s"]) / wall, 3),
-        "requests": args.requests,
-        "qps": args.qps,
-        "offered_samples_per_s_M": round(sum(r["width"] * r["height"] * r["spp"] for r in reqs) / reqs[-1]["arrival"] / 1e6, 2),
-        "steps": s1["steps"] - s0["steps"],
-        "wall_s": round(wall, 3),
-        "n_gpus": 1,
-        "config": {"workload": "4 resident instanced scenes (1024-4096 instances), Poisson arrivals, mixed "
-                               "resolution and spp, 8 bounces", "max_paths_per_step": args.max_paths,
-                   "hbm_budget_gb": 200},
-    }
-    print(json.dumps(out), flush=True)
+    # warm-up: two small requests so first-use costs stay out of the timed runs
+    warm = serving.mixed_requests(2, 20.0, 4, seed=99)
+    serving.run_open_loop(srv, [dict(r, arrival=0.0) for r in warm], camera)
+    out = []
+    for qps in qps_list:
+        reqs = serving.mixed_requests(requests, qps, 4, seed=0)
+        s0 = srv.stats()
+        lat, wall, _ = serving.run_open_loop(srv, reqs, camera)
+        s1 = srv.stats()
+        out.append({
+            "qps": qps,
+            "requests": requests,
+            "p50_ms": round(float(np.percentile(lat, 50)) * 1e3, 2),
+            "p99_ms": round(float(np.percentile(lat, 99)) * 1e3, 2),
+            "mean_ms": round(float(lat.mean()) * 1e3, 2),
+            "aggregate_mrays_s": round((s1["rays"] - s0["rays"]) / wall / 1e6, 2),
+            "gpu_busy_frac": round((s1["gpu_busy_s"] - s0["gpu_busy_s"]) / wall, 3),
+            "offered_samples_per_s_M": round(sum(r["width"] * r["height"] * r["spp"] for r in reqs) / reqs[-1]["arrival"] / 1e6, 2),
+            "steps": s1["steps"] - s0["steps"],
+            "wall_s": round(wall, 3),
+        })
+    srv.close()
+    del keep
+    ctx.close()
+    return out
+
+
+SERVE_CONFIG = {"workload": "4 resident instanced scenes (1024-4096 instances), Poisson arrivals, mixed resolution "
+                            "(256^2..1920x1080) and spp (1..64), 8 bounces", "hbm_budget_gb": 200}
+
+
+def serve_bench(args):
+    """The serving workload at --qps (a comma-separated list gives a sweep, one line each)."""
+    for r in serve_streams([float(q) for q in str(args.qps).split(",")], args.requests, args.max_paths):
+        line = {"metric": "serving request latency p50/p99 and aggregate Mrays/s", "value": r["p99_ms"],
+                "unit": "ms (p99 latency)", "higher_is_better": False, "n_gpus": 1}
+        line.update(r)
+        line["config"] = dict(SERVE_CONFIG, max_paths_per_step=args.max_paths)
+        print(json.dumps(line), flush=True)
+
+
+def small_workload_rate(frames=16):
+    """The small workload (64 icospheres in a box, 256x256, 8 bounces) at `frames` samples per
+    pixel in one nb_render_frames call (sample-parallel): Mrays/s over an untimed counted pass and
+    a timed one of the same frames."""
+    import nbounce_amd as nb
+    from nbounce_amd import scenegen
+    P, desc, scene, arrays, _ = build_workload("small")
+    blas, tlas, inst = arrays
+    ctx = nb.Context(0)
+    sb = nb.SceneBuffers(ctx, blas, tlas, inst, scene.vertices, scene.indices)
+    env = nb.EnvMap(ctx, scenegen.sky_cube(P["env_face"]))
+    cam = nb.CameraController(world_position=P["eye"], target=P["target"], aspect_ratio=1.0)
+    pt = nb.Pathtracer(ctx, sb, cam, env, P["width"], P["height"], resolution_factor=1.0)
+    pt.globals.bounces, pt.globals.contribution_factor = P["bounces"], P["contribution_factor"]
+    pt.dispatch_frames(frames)                    # warm-up
+    pt.invalidate()
+    ctx.reset_counters()
+    pt.dispatch_frames(frames)
+    ctx.synchronize()
+    c = ctx.counters()
+    pt.invalidate()
+    ctx.enable_counters(False)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    pt.dispatch_frames(frames)
+    ctx.synchronize()
+    wall = time.perf_counter() - t0
+    ctx.close()
+    return {"value": round(c["rays"] / wall / 1e6, 2), "unit": "Mrays/s", "ms_per_image": round(wall * 1e3, 3),
+            "rays": int(c["rays"]),
+            "config": {"workload": f"{desc}, {P['width']}x{P['height']}, {frames} spp in one call, {P['bounces']} bounces"}}
 
 
 def serve_cache_bench(args):
@@ -187,7 +239,8 @@ def serve_cache_bench(args):
         srv.set_cache(idx, cache)
         keep.append(sb)
         caches.append(cache)
-    reqs = serving.mixed_requests(args.requests, args.qps, 4, seed=0)
+    qps = float(str(args.qps).split(",")[0])
+    reqs = serving.mixed_requests(args.requests, qps, 4, seed=0)
 
     def camera(r):
         return nb.CameraController(world_position=r["eye"], target=r["target"],
@@ -255,7 +308,7 @@ def serve_cache_bench(args):
         "aggregate_mrays_s": round((plain_rays + cc["rays"]) / wall / 1e6, 2),
         "gpu_busy_frac": round((s1["gpu_busy_s"] - s0["gpu_busy_s"]) / wall, 3),
         "requests": args.requests,
-        "qps": args.qps,
+        "qps": qps,
         "wall_s": round(wall, 3),
         "n_gpus": 1,
         "config": {"workload": "4 resident instanced scenes (1024-4096 instances) with per-scene caches "
@@ -391,8 +444,10 @@ def main():
     ap.add_argument("--lr", type=float, default=1e-3, help="cache: Adam learning rate")
     ap.add_argument("--max-batch", type=int, default=0,
                     help="cache: training records per frame (0: all, three per training path)")
-    ap.add_argument("--qps", type=float, default=20.0)
+    ap.add_argument("--qps", default="20", help="serve: Poisson arrival rate(s), comma-separated for a sweep")
     ap.add_argument("--requests", type=int, default=200)
+    ap.add_argument("--no-secondary", action="store_true",
+                    help="render: skip the secondary workloads (small scene, serving stream, learned cache)")
     ap.add_argument("--max-paths", type=int, default=1 << 23)
     ap.add_argument("--count-in-timed", action="store_true", help="keep the kernel counters on in the timed pass")
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
@@ -457,6 +512,13 @@ def main():
     dev_ms = ctx.timer_stop()
     ctx.synchronize()
     wall = time.perf_counter() - t0
+    # the same frames again in one nb_render_frames call (the image is the same bit for bit)
+    pt.invalidate()
+    ctx.synchronize()
+    t1 = time.perf_counter()
+    pt.dispatch_frames(args.steps)
+    ctx.synchronize()
+    wall_one_call = time.perf_counter() - t1
     ctx.enable_counters(True)
     launches = args.steps if F == 1 else math.ceil(args.steps / F)
     rays = c["rays"]
@@ -477,6 +539,29 @@ def main():
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(P, scene, arrays, env_faces, pt._camera_data, pt.table, target_s=args.cpu_seconds)
+
+    # the HBM traffic the counters measured, as a fraction of peak at this kernel time
+    hbm_frac = round(traffic / (kernel_ms * 1e-3) / (PEAK_HBM_GBS * 1e9), 4) if traffic else None
+
+    secondary = None
+    if not args.no_secondary and args.workload == "large":
+        del pt, sb, env
+        ctx.close()
+        secondary = {
+            "plain_one_call": {
+                "value": round(rays / wall_one_call / 1e6, 3), "unit": "Mrays/s",
+                "ms_per_image": round(wall_one_call * 1e3, 3),
+                "config": {"workload": f"the same {args.steps} frames of the same workload in one nb_render_frames "
+                                       "call (the same image bit for bit)"}},
+            "small": small_workload_rate(16),
+            "serving": dict(serve_streams([20.0], 100, args.max_paths)[0], config=SERVE_CONFIG),
+        }
+        c4 = cache_equal_error(1 << 16, 1e-3)
+        secondary["learned_cache"] = {
+            "value": c4["ratio"], "unit": "equal relative-MSE frame time, cache-assisted / plain (lower is better)",
+            "with_training_ratio": c4["with_training_ratio"], "cache_1spp": c4["cache_1spp"],
+            "plain_spp_at_equal_error": c4["plain_spp_at_equal_error"], "plain_ms_per_spp": c4["plain_ms_per_spp"],
+            "config": c4["config"]}
 
     out = {
         "metric": "Mrays/s at 1080p" if args.workload == "large" else "Mrays/s at 256x256",
@@ -503,7 +588,13 @@ def main():
             "samples_per_s_M": round(c["samples"] / wall / 1e6, 3),
         },
         "roofline": {
+            # the ceiling frac is priced against (no matrix work); the counters show HBM itself is
+            # far from saturated: the limiter is the latency of the per-lane node fetches through
+            # the texture-data path, with VALU issue beside it
             "bound": "hbm",
+            "limiter": "dependent node fetches through the texture-data path (TD busy 0.82, 40 % of it stalled "
+                       "on the L1) and VALU issue (0.59 busy); HBM counter traffic is hbm_counter_frac of peak",
+            "hbm_counter_frac": hbm_frac,
             "achieved": round(achieved, 2),
             "peak": PEAK_HBM_GBS,
             "unit": "GB/s",
@@ -517,6 +608,8 @@ def main():
         },
         "cpu_baseline": cpu,
     }
+    if secondary is not None:
+        out["secondary"] = secondary
     print(json.dumps(out), flush=True)
 
 
