@@ -24,14 +24,17 @@ sys.path.insert(0, ROOT)
 PEAK_HBM_GBS = 8000.0
 
 
-def algorithmic_bytes(c):
+def algorithmic_bytes(c, pipelined=False):
     """Bytes the traversal and shading must read/write: 32 per box test, 48 per triangle test,
     64 per instance entry, 304 per hit (3 x 48-B vertices + 160-B instance), 64 per miss (four
-    RGBA32F texels), 32 per ray (two Sobol entries), 48 per sample (jitter entry + RGBA32F
-    read-modify-write)."""
+    RGBA32F texels), 32 per ray (two Sobol entries), per sample the 16-B jitter entry and the
+    RGBA32F read-modify-write of the target (48 B), or, for the path kernel of a pipelined
+    nb_render, the 16-B jitter entry and the sample's 16-B write (32 B; the mix kernel moves
+    the other 48 B per pixel)."""
     misses = c["rays"] - c["hits"]
+    per_sample = 32 if pipelined else 48
     return (32 * c["n_aabb"] + 48 * c["n_tri"] + 64 * c["n_inst"] + 304 * c["hits"] + 64 * misses
-            + 32 * c["rays"] + 48 * c["samples"])
+            + 32 * c["rays"] + per_sample * c["samples"])
 
 
 def build_workload(name, share_blas=True):
@@ -524,13 +527,14 @@ def main():
     rays = c["rays"]
     value = rays / wall / 1e6
     kernel_ms = dev_ms / launches
-    bytes_per_launch = algorithmic_bytes(c) / launches
+    pipelined = F == 1 and os.environ.get("NB_PIPELINE", "1") != "0"
+    bytes_per_launch = algorithmic_bytes(c, pipelined) / launches
     achieved = bytes_per_launch / (kernel_ms * 1e-3) / 1e9
 
     traffic = None
     try:
         with open(os.path.join(ROOT, "profiles", "traffic.json")) as f:
-            key = "render_pt_kernel instanced 1920x1080 12 bounces shared BLAS"
+            key = "render_pt_kernel instanced 1920x1080 12 bounces shared BLAS" + (" pipelined" if pipelined else "")
             if args.workload == "large" and not args.reference_blas and F == 1:
                 traffic = json.load(f).get(key, {}).get("hbm_bytes_per_launch")
     except OSError:
@@ -581,6 +585,7 @@ def main():
                         f"{args.steps} spp (1 spp per step), {P['bounces']} bounces",
             "blas": "per instance (reference)" if args.reference_blas else "shared per mesh",
             "frames_per_launch": F,
+            "pipelined": pipelined,
             "rays": int(rays),
             "rays_per_sample": round(rays / max(1, c["samples"]), 4),
             "per_ray": {k: round(c[k] / max(1, rays), 3) for k in ("n_aabb", "n_tri", "n_inst", "hits")},
@@ -604,6 +609,9 @@ def main():
                               "(profiles/render_pt_kernel_instanced_1080p_traffic.md)" if traffic else None,
             "kernel": "render_pt_kernel" if os.environ.get("NB_RENDER", "pt") != "tile" else "render_kernel",
             "kernel_ms": round(kernel_ms, 4),
+            "kernel_ms_is": ("the timed region / launches: pipelined path kernels of consecutive frames overlap, "
+                             "so each launch's own duration is longer; the mix kernels are inside")
+                            if pipelined else "the timed region / launches",
             "algorithmic_bytes_per_launch": int(bytes_per_launch),
         },
         "cpu_baseline": cpu,

@@ -219,9 +219,16 @@ nb_status nb_image_write_exr(const char *path, const float *rgba, uint32_t width
 
 /* ---- rendering (Pathtracer::dispatch, pathtracer.rs:225-239 -> pathtracing.wgsl:main) ----
  * nb_render: one sample per pixel of frame globals->sample with weight globals->weight,
- * color = mix(previous, sample, weight) (pathtracing.wgsl:247-260).
+ * color = mix(previous, sample, weight) (pathtracing.wgsl:247-260).  The frame's path kernel
+ * runs on one of two library-owned side streams and writes the frame's samples; the mix runs on
+ * the context stream after it, so work queued on the context stream after nb_render (and every
+ * wait on it) follows the whole frame, while consecutive frames' paths overlap.  The target
+ * keeps two sample buffers (32 bytes per pixel) from its first nb_render on.  NB_PIPELINE=0 in
+ * the environment renders on the context stream alone (the same image bit for bit).
  * nb_render_frames: frames first->sample .. first->sample + n_frames - 1 in one launch, each
- * with weight 1/sample; the image equals n_frames consecutive nb_render calls bit for bit. */
+ * with weight 1/sample; the image equals n_frames consecutive nb_render calls bit for bit.
+ * Targets of up to 2^19 pixels render one path per (frame, pixel) with the serving kernels;
+ * larger ones keep a pixel in one lane for every frame. */
 nb_status nb_render(nb_context *ctx, const nb_scene *scene, nb_target *target, const nb_camera *camera,
                     const nb_globals *globals, const nb_envmap *env);
 nb_status nb_render_frames(nb_context *ctx, const nb_scene *scene, nb_target *target, const nb_camera *camera,

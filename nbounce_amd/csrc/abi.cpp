@@ -446,6 +446,77 @@ nb_status render_frames_sample_parallel(nb_context *ctx, const nb_scene *scene, 
     return NB_OK;
 }
 
+// ---- pipelined nb_render -----------------------------------------------------------------------
+// One nb_render launch leaves CUs idle while its last, longest paths finish.  Consecutive frames'
+// paths are independent; only the mix into the target depends on the previous frame.  So each
+// frame's path kernel runs on one of two side streams and writes the frame's samples to one of
+// the target's two sample buffers, and a mix kernel on the context stream applies the
+// accumulation step in frame order.  A path kernel waits for the context stream as it was just
+// before the previous frame's mix was enqueued: that includes every earlier mix (the last reader
+// of the sample buffer it writes), not the previous frame's paths, so the two frames' paths run
+// side by side.  When work on the context stream since then changed the work-unit order or reset
+// the counters, it waits for all of the context stream instead.  The image is the same bit for
+// bit (the mix is the same binary32 expression).  NB_PIPELINE=0 renders on the context stream,
+// one launch after another.
+bool pipeline_enabled() {
+    const char *e = std::getenv("NB_PIPELINE");
+    return !(e && std::strcmp(e, "0") == 0);
+}
+
+nb_status pipe_setup(nb_context *ctx, nb_target *t) {
+    if (!ctx->pipe_gate) {
+        for (int i = 0; i < 2; ++i) {
+            NB_HIP(hipStreamCreateWithFlags(&ctx->pipe_stream[i], hipStreamNonBlocking));
+            NB_HIP(hipMalloc(&ctx->pipe_work[i], kWorkBytes));
+            NB_HIP(hipMemset(ctx->pipe_work[i], 0, kWorkBytes));
+        }
+        NB_HIP(hipEventCreateWithFlags(&ctx->pipe_done, hipEventDisableTiming));
+        NB_HIP(hipEventCreateWithFlags(&ctx->pipe_gate, hipEventDisableTiming));
+    }
+    if (!t->samples[0]) {
+        const size_t bytes = static_cast<size_t>(t->width) * t->height * 16;
+        for (int i = 0; i < 2; ++i) {
+            hipError_t e = hipMalloc(&t->samples[i], bytes);
+            if (e != hipSuccess) {
+                hipFree(t->samples[0]);
+                t->samples[0] = t->samples[1] = nullptr;
+                return set_error(e == hipErrorOutOfMemory ? NB_ERR_OUT_OF_MEMORY : NB_ERR_HIP,
+                                 "sample buffers: %s", hipGetErrorString(e));
+            }
+        }
+    }
+    return NB_OK;
+}
+
+nb_status render_pipelined(nb_context *ctx, const nb_scene *scene, nb_target *t, const nb_camera *cam,
+                           const nb_globals *g, RenderParams &p) {
+    NB_TRY(pipe_setup(ctx, t));
+    NB_TRY(unit_order_prepare(t, scene, cam, g, p, ctx->stream));
+    const uint32_t slot = static_cast<uint32_t>(ctx->pipe_seq & 1u);
+    hipStream_t side = ctx->pipe_stream[slot];
+    if (!ctx->pipe_gate_valid || ctx->pipe_fence || t->order_dirty) {
+        NB_HIP(hipEventRecord(ctx->pipe_gate, ctx->stream));
+        ctx->pipe_gate_valid = true;
+        ctx->pipe_fence = false;
+        t->order_dirty = false;
+    }
+    NB_HIP(hipStreamWaitEvent(side, ctx->pipe_gate, 0));
+    p.sample_out = t->samples[slot];
+    p.work = ctx->pipe_work[slot];
+    if (ctx->render_timing) NB_HIP(hipEventRecord(ctx->ev_start, side));
+    NB_TRY(launch_render(p, side));
+    if (ctx->render_timing) {
+        NB_HIP(hipEventRecord(ctx->ev_stop, side));
+        ctx->timed = true;
+    }
+    NB_HIP(hipEventRecord(ctx->pipe_done, side));
+    NB_HIP(hipEventRecord(ctx->pipe_gate, ctx->stream));   // before this frame's mix
+    NB_HIP(hipStreamWaitEvent(ctx->stream, ctx->pipe_done, 0));
+    NB_TRY(launch_mix_samples(t->pixels, p.sample_out, t->width * t->height, p.first_weight, ctx->stream));
+    ++ctx->pipe_seq;
+    return unit_order_update(t, p, 1u, ctx->stream);
+}
+
 nb_status timed_frames(nb_context *ctx, const RenderParams &p) {
     // event records between launches cost the stream about 0.5 % at 1080p, so they are made only
     // when the caller asked for per-render times
@@ -505,6 +576,15 @@ nb_status nb_context_create(int device, nb_context **out) {
 static void release_context(nb_context *ctx) {
     hipSetDevice(ctx->device);
     if (ctx->stream) hipStreamSynchronize(ctx->stream);
+    for (int i = 0; i < 2; ++i) {
+        if (ctx->pipe_stream[i]) {
+            hipStreamSynchronize(ctx->pipe_stream[i]);
+            hipStreamDestroy(ctx->pipe_stream[i]);
+        }
+        hipFree(ctx->pipe_work[i]);
+    }
+    if (ctx->pipe_gate) hipEventDestroy(ctx->pipe_gate);
+    if (ctx->pipe_done) hipEventDestroy(ctx->pipe_done);
     hipFree(ctx->sobol);
     hipFree(ctx->counters);
     hipFree(ctx->work);
@@ -725,6 +805,12 @@ nb_status nb_target_destroy(nb_target *t) {
     if (!t) return NB_OK;
     hipSetDevice(t->ctx->device);
     hipStreamSynchronize(t->ctx->stream);
+    for (int i = 0; i < 2; ++i) {
+        // (each path kernel's mix follows it on the context stream; this also covers a path
+        // kernel whose mix was never enqueued)
+        if (t->ctx->pipe_stream[i]) hipStreamSynchronize(t->ctx->pipe_stream[i]);
+        hipFree(t->samples[i]);
+    }
     hipFree(t->pixels);
     unit_order_free(t);
     nb_context *ctx = t->ctx;
@@ -761,6 +847,7 @@ nb_status nb_render(nb_context *ctx, const nb_scene *scene, nb_target *target, c
     NB_TRY(check_render_args(ctx, scene, target, camera, globals, env));
     NB_TRY(use_device(ctx));
     RenderParams p = make_params(ctx, scene, target, camera, globals, env);
+    if (pipeline_enabled()) return render_pipelined(ctx, scene, target, camera, globals, p);
     NB_TRY(unit_order_prepare(target, scene, camera, globals, p, ctx->stream));
     NB_TRY(timed_frames(ctx, p));
     return unit_order_update(target, p, 1u, ctx->stream);
@@ -849,6 +936,7 @@ nb_status nb_counters_reset(nb_context *ctx) {
     if (!ctx) return set_error(NB_ERR_INVALID_ARGUMENT, "null context");
     NB_TRY(use_device(ctx));
     NB_HIP(hipMemsetAsync(ctx->counters, 0, kCounterBytes, ctx->stream));
+    ctx->pipe_fence = true;   // the next pipelined path kernel counts after the reset
     return NB_OK;
 }
 

@@ -103,6 +103,8 @@ struct RenderParams {
     uint32_t grab_units;     // work units per grab before that
     const uint32_t *unit_order;  // position -> work unit, a permutation within each XCD band, or null
     uint32_t *unit_cost;     // rays traced per work unit, added as pixels finish, or null
+    float4 *sample_out;      // or null; else (n_frames 1) each pixel's sample goes here, the target
+                             // is neither read nor written
 };
 
 // ---- serving (serve.hip, server.cpp) ----------------------------------------------------------
@@ -164,6 +166,10 @@ nb_status serve_launch_accumulate(const ServeStep *step, uint32_t grid, hipStrea
 // Kernel launchers (trace.hip).
 nb_status launch_render(const RenderParams &p, void *stream);
 nb_status launch_primary_hits(const RenderParams &p, void *stream);
+// target[i] = mix(target[i] (0 when weight is not positive), samples[i], weight) for n pixels: the
+// step of the render kernels' accumulation, as a kernel of its own for frames whose paths were
+// rendered into a sample buffer (RenderParams::sample_out).
+nb_status launch_mix_samples(float4 *target, const float4 *samples, uint32_t n, float weight, hipStream_t s);
 
 }  // namespace nb
 
@@ -186,6 +192,17 @@ struct nb_context {
     void *sp_block = nullptr;
     float4 *sp_results = nullptr;
     uint32_t sp_max_paths = 0;
+    // Pipelined nb_render (abi.cpp): each frame's path kernel runs on one of two side streams and
+    // writes the frame's samples; the mix into the target runs on `stream`.  Created at first use.
+    hipStream_t pipe_stream[2] = {nullptr, nullptr};
+    uint32_t *pipe_work[2] = {nullptr, nullptr};   // work counters of each side stream's launches
+    hipEvent_t pipe_gate = nullptr;   // `stream` as it was before the last mix was enqueued
+    hipEvent_t pipe_done = nullptr;   // end of the last path kernel
+    uint64_t pipe_seq = 0;            // pipelined frames so far (the side stream alternates)
+    bool pipe_gate_valid = false;
+    // Set when work on `stream` since the last mix changed what a path kernel reads or adds to
+    // (the counters): the next path kernel then waits for all of `stream`.
+    bool pipe_fence = false;
     // Scenes, targets and environments keep their context alive: nb_context_destroy only marks
     // it, and the last child's destroy releases it.
     uint32_t children = 0;
@@ -218,6 +235,10 @@ struct nb_target {
     nb_camera cost_camera{};
     uint32_t cost_bounces = 0;
     float cost_contribution = 0.0f;
+    // set whenever order.hip enqueues a kernel that rewrites unit_cost or unit_order
+    bool order_dirty = false;
+    // Pipelined nb_render: the two frames' sample buffers (RGBA32F per pixel), at first use.
+    float4 *samples[2] = {nullptr, nullptr};
 };
 
 namespace nb {

@@ -122,6 +122,7 @@ nb_status unit_order_prepare(nb_target *t, const nb_scene *scene, const nb_camer
         hipLaunchKernelGGL(shift_kernel, dim3((t->n_units + 255u) / 256u), dim3(256), 0, s, t->unit_cost, t->n_units,
                            shift);
         NB_ORDER_HIP(hipGetLastError());
+        t->order_dirty = true;
         t->cost_frames = same_paths ? t->cost_frames / 2u : 0u;
         t->view_frames = 0;
         if (!same_paths) t->order_valid = false;
@@ -157,6 +158,7 @@ nb_status unit_order_update(nb_target *t, const RenderParams &p, uint32_t n_fram
     NB_ORDER_HIP(hipcub::DeviceSegmentedRadixSort::SortPairsDescending(
         t->sort_tmp, tmp, t->sort_keys, t->sort_keys + n, t->sort_vals, t->unit_order, static_cast<int>(n),
         static_cast<int>(kBands), t->sort_offsets, t->sort_offsets + 1, 0, 8, s));
+    t->order_dirty = true;
     t->order_valid = true;
     return NB_OK;
 }

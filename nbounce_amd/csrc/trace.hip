@@ -50,19 +50,20 @@ __global__ __launch_bounds__(64) void render_kernel(RenderParams p) {
     pc.bounces = p.bounces;
     pc.contribution_factor = p.contribution_factor;
     pc.shift = hash4f(x, y, x, y);
-    F3 col = f3(0.0f, 0.0f, 0.0f);
+    F3 col = f3(0.0f, 0.0f, 0.0f), s = f3(0.0f, 0.0f, 0.0f);
     for (uint32_t f = 0; f < p.n_frames; ++f) {
         const uint32_t sample = p.first_sample + f;
         const float w = (f == 0 && !p.weights_from_sample) ? p.first_weight : 1.0f / static_cast<float>(sample);
-        if (f == 0 && w > 0.0f) { float4 prev = *px; col = f3(prev.x, prev.y, prev.z); }
+        if (f == 0 && w > 0.0f && !p.sample_out) { float4 prev = *px; col = f3(prev.x, prev.y, prev.z); }
         pc.sample = sample;
         float4 jit = sobol_fetch(pc.sobol, sample * p.sobol_stride, pc.shift);
         Ray ray = camera_ray(p.c2w, p.width, p.height, x, y, jit.x, jit.y);
-        F3 s = sample_path<K>(pc, ray, st, c);
+        s = sample_path<K>(pc, ray, st, c);
         // mix(color, s, weight) = color * (1 - weight) + s * weight
         col = f3(col.x * (1.0f - w) + s.x * w, col.y * (1.0f - w) + s.y * w, col.z * (1.0f - w) + s.z * w);
     }
-    *px = make_float4(col.x, col.y, col.z, 1.0f);
+    if (p.sample_out) p.sample_out[static_cast<size_t>(y) * p.width + x] = make_float4(s.x, s.y, s.z, 1.0f);
+    else *px = make_float4(col.x, col.y, col.z, 1.0f);
     flush_counters(p.counters, c, p.n_frames);
 }
 
@@ -181,7 +182,9 @@ __global__ __launch_bounds__(64) void render_pt_kernel(RenderParams p) {
                 bounce = 0;
                 ++frames_done;
                 if (++f == p.n_frames) {
-                    p.target[static_cast<size_t>(y) * p.width + x] = make_float4(col.x, col.y, col.z, 1.0f);
+                    const size_t i = static_cast<size_t>(y) * p.width + x;
+                    if (p.sample_out) p.sample_out[i] = make_float4(out.x, out.y, out.z, 1.0f);
+                    else p.target[i] = make_float4(col.x, col.y, col.z, 1.0f);
                     mode = kIdle;
                     if (p.unit_cost) {
                         const uint32_t tile = (y >> 3) * p.tiles_x + (x >> 3), k = ((y & 7u) << 3) | (x & 7u);
@@ -225,7 +228,7 @@ __global__ __launch_bounds__(64) void render_pt_kernel(RenderParams p) {
                     col = f3(0.0f, 0.0f, 0.0f);
                     // first frame: the caller's weight (nb_render) or 1/sample (nb_render_frames)
                     w = !p.weights_from_sample ? p.first_weight : 1.0f / static_cast<float>(p.first_sample);
-                    if (w > 0.0f) {
+                    if (w > 0.0f && !p.sample_out) {
                         const float4 prev = p.target[static_cast<size_t>(y) * p.width + x];
                         col = f3(prev.x, prev.y, prev.z);
                     }
@@ -273,6 +276,22 @@ __global__ __launch_bounds__(64) void render_pt_kernel(RenderParams p) {
         }
     }
     if (kCount) flush_counters(p.counters, c, frames_done);   // else the counts are dead code
+}
+
+// One step of the accumulation (main, pathtracing.wgsl:243-260) for a frame rendered into a
+// sample buffer: the same expression, in the same binary32 order, as the render kernels' mix.
+__global__ __launch_bounds__(256) void mix_samples_kernel(float4 *target, const float4 *samples, uint32_t n, float w) {
+    const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+    if (i >= n) return;
+    const float4 s = samples[i];
+    F3 col = f3(0.0f, 0.0f, 0.0f);
+    if (w > 0.0f) {
+        const float4 prev = target[i];
+        col = f3(prev.x, prev.y, prev.z);
+    }
+    // mix(color, s, weight) = color * (1 - weight) + s * weight
+    col = f3(col.x * (1.0f - w) + s.x * w, col.y * (1.0f - w) + s.y * w, col.z * (1.0f - w) + s.z * w);
+    target[i] = make_float4(col.x, col.y, col.z, 1.0f);
 }
 
 __global__ void reset_work_kernel(uint32_t *work, uint32_t first) {
@@ -439,6 +458,14 @@ nb_status launch_render(const RenderParams &p, void *stream) {
 
 nb_status launch_primary_hits(const RenderParams &p, void *stream) {
     return launch_with_k<HitsKernel>(p, stream, "primary-hit");
+}
+
+nb_status launch_mix_samples(float4 *target, const float4 *samples, uint32_t n, float weight, hipStream_t s) {
+    if (n == 0) return NB_OK;
+    hipLaunchKernelGGL(mix_samples_kernel, dim3((n + 255u) / 256u), dim3(256), 0, s, target, samples, n, weight);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return set_error(NB_ERR_HIP, "mix launch: %s", hipGetErrorString(e));
+    return NB_OK;
 }
 
 }  // namespace nb

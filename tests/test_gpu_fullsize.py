@@ -6,10 +6,11 @@ XCD band, the work stealing at the tail and many waves per CU are in play.
   error <= 1e-3, bit-identical fraction printed);
 * the persistent kernel and the one-tile-per-wave kernel render the same image bit for bit
   (two independent schedules of the same samples);
-* 4 frames in one launch equal 4 launches bit for bit;
+* 4 frames in one launch equal 4 (pipelined) launches bit for bit;
 * the counted and the counter-free kernel render the same image, and the counts are the same
   in two passes (a checksum of the work done);
 * cost-ordered work units (rebuilt after 1, 2 and 4 frames) and band order render the same image.
+* pipelined frames (side streams + ordered mix) equal launches on the context stream.
 """
 import os
 
@@ -131,3 +132,18 @@ def test_fullsize_unit_order_changes_nothing(full):
     b = _render(_tracer(nb, ctx, sb, env, cam, P), 6, env={"NB_UNIT_ORDER": "0"})
     assert np.isfinite(a).all()
     np.testing.assert_array_equal(a.view(np.uint32), b.view(np.uint32))
+
+
+def test_fullsize_pipelined_equals_sequential(full):
+    """Pipelined nb_render (path kernels of consecutive frames side by side, the mix on the
+    context stream) against one launch after another on the context stream."""
+    nb, ctx, sb, env, cam, osc, faces, P = full
+    print("case: 1920x1080, 6 frames, pipelined vs sequential nb_render", flush=True)
+    counts, images = [], []
+    for pipe in ("1", "0"):
+        ctx.reset_counters()
+        images.append(_render(_tracer(nb, ctx, sb, env, cam, P), 6, env={"NB_PIPELINE": pipe}))
+        counts.append(ctx.counters())
+    assert np.isfinite(images[0]).all()
+    np.testing.assert_array_equal(images[0].view(np.uint32), images[1].view(np.uint32))
+    assert counts[0] == counts[1]
