@@ -180,6 +180,37 @@ def serve_bench(args):
         print(json.dumps(line), flush=True)
 
 
+def view_rate(nb, ctx, sb, env, P, eye, target, frames=32):
+    """Mrays/s of pipelined 1-spp nb_render calls at 1920x1080 from another camera: an untimed
+    counted pass over the frames gives the rays, a timed pass renders them again."""
+    cam = nb.CameraController(world_position=eye, target=target, aspect_ratio=P["width"] / P["height"])
+    pt = nb.Pathtracer(ctx, sb, cam, env, P["width"], P["height"], resolution_factor=1.0)
+    pt.globals.bounces = P["bounces"]
+    pt.globals.contribution_factor = P["contribution_factor"]
+    ctx.enable_counters(True)
+    ctx.reset_counters()
+    for _ in range(frames):
+        pt.dispatch()
+    ctx.synchronize()
+    c = ctx.counters()
+    pt.invalidate()
+    ctx.enable_counters(False)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(frames):
+        pt.dispatch()
+    ctx.synchronize()
+    wall = time.perf_counter() - t0
+    ctx.enable_counters(True)
+    rays = c["rays"]
+    return {"value": round(rays / wall / 1e6, 3), "unit": "Mrays/s", "ms_per_frame": round(wall * 1e3 / frames, 4),
+            "rays_per_sample": round(rays / max(1, c["samples"]), 4),
+            "per_ray": {k: round(c[k] / max(1, rays), 3) for k in ("n_aabb", "n_tri", "n_inst", "hits")},
+            "config": {"workload": f"the instanced scene, {P['width']}x{P['height']}, {P['bounces']} bounces, "
+                                   f"{frames} frames of 1 spp, camera at {tuple(eye)} looking at {tuple(target)} "
+                                   "(inside the instance cloud)"}}
+
+
 def small_workload_rate(frames=16):
     """The small workload (64 icospheres in a box, 256x256, 8 bounces) at `frames` samples per
     pixel in one nb_render_frames call (sample-parallel): Mrays/s over an untimed counted pass and
@@ -549,9 +580,11 @@ def main():
 
     secondary = None
     if not args.no_secondary and args.workload == "large":
+        interior = view_rate(nb, ctx, sb, env, P, scenegen.LARGE_INTERIOR["eye"], scenegen.LARGE_INTERIOR["target"])
         del pt, sb, env
         ctx.close()
         secondary = {
+            "interior_view": interior,
             "plain_one_call": {
                 "value": round(rays / wall_one_call / 1e6, 3), "unit": "Mrays/s",
                 "ms_per_image": round(wall_one_call * 1e3, 3),

@@ -161,6 +161,23 @@ nb_status nb_sobol_set(nb_context *ctx, const float *table, uint32_t rows, uint3
  * texels: 6 faces in Vulkan layer order +X,-X,+Y,-Y,+Z,-Z, each face_size^2 RGBA32F, row-major. */
 nb_status nb_envmap_create_cube(nb_context *ctx, const float *texels, uint32_t face_size, nb_envmap **out);
 nb_status nb_envmap_destroy(nb_envmap *env);
+/* An equirectangular (lat-long) RGBA32F map, height rows of width texels, row 0 at the top.  No
+ * counterpart in the reference, which samples only the cube (pathtracing.wgsl:14,168); this is
+ * the native form of a 512x256-style sky.  Direction d (y up) looks up u = atan2(d.z, d.x)/2pi +
+ * 1/2 (+x at u = 1/2, +z at 3/4, -z at 1/4), v = atan2(|d.xz|, d.y)/pi (+y at row 0), bilinear
+ * at LOD 0, wrapping across the u seam and clamping at the poles.  Accepted wherever a cube is
+ * (nb_render*, nb_server_add_scene, nb_render_cached). */
+nb_status nb_envmap_create_latlong(nb_context *ctx, const float *texels, uint32_t width, uint32_t height,
+                                   nb_envmap **out);
+
+/* ---- render options (no counterpart in the reference) ------------------------------------
+ * roulette_first_bounce: Russian roulette (pathtracing.wgsl:227-236) applies from this bounce
+ * on; paths continue unconditionally, throughput unchanged, before it.  0 (the default) is the
+ * reference's rule; 3 is "Russian roulette from bounce 3".  Applies to nb_render and
+ * nb_render_frames on the context from the next call; served and cache-assisted renders keep
+ * the reference's rule. */
+typedef struct { uint32_t roulette_first_bounce; } nb_render_options;
+nb_status nb_render_options_set(nb_context *ctx, const nb_render_options *options);
 /* EnvMap::load (src/pathtracing/envmap.rs:12 -> Texture::create_cubemap, src/common/texture.rs:89-141):
  * a cube DDS file, decoded on the host to RGBA32F (mip 0 of each face; the shader samples LOD 0,
  * pathtracing.wgsl:168) and uploaded as nb_envmap_create_cube does.  Formats: BC6H_UF16 (the

@@ -38,6 +38,8 @@ SIGNATURES = {
     "nb_sobol_generate": [_u32, _u32, _u32, _p],
     "nb_sobol_set": [_p, _p, _u32, _u32, _u32],
     "nb_envmap_create_cube": [_p, _p, _u32, _c.POINTER(_p)],
+    "nb_envmap_create_latlong": [_p, _p, _u32, _u32, _c.POINTER(_p)],
+    "nb_render_options_set": [_p, _p],
     "nb_envmap_destroy": [_p],
     "nb_envmap_load": [_p, _c.c_char_p, _c.POINTER(_p)],
     "nb_dds_read_cube": [_c.c_char_p, _p, _c.c_uint64, _ptr(_u32)],

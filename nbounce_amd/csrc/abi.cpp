@@ -339,7 +339,7 @@ RenderParams make_params(nb_context *ctx, const nb_scene *scene, nb_target *t, c
                          const nb_globals *g, const nb_envmap *env) {
     RenderParams p{};
     p.scene = scene->dev;
-    p.env = DevEnv{env ? env->texels : nullptr, env ? env->n : 0u};
+    p.env = dev_env(env);
     p.target = t->pixels;
     p.width = t->width;
     p.height = t->height;
@@ -359,6 +359,7 @@ RenderParams make_params(nb_context *ctx, const nb_scene *scene, nb_target *t, c
     p.tiles_per_xcd = (p.n_tiles + 7) / 8;
     p.hits_out = nullptr;
     p.work = ctx->work;
+    p.rr_first = ctx->rr_first;
     return p;
 }
 
@@ -407,7 +408,7 @@ nb_status render_frames_sample_parallel(nb_context *ctx, const nb_scene *scene, 
     for (uint32_t done = 0; done < n_frames;) {
         const uint32_t count = std::min(per_launch, n_frames - done);
         SpDesc h{};
-        h.scene = ServeSceneDev{scene->dev, DevEnv{env->texels, env->n}};
+        h.scene = ServeSceneDev{scene->dev, dev_env(env)};
         std::memcpy(h.request.c2w, cam->clip_to_world, sizeof(h.request.c2w));
         h.request.target = t->pixels;
         h.request.width = t->width;
@@ -415,6 +416,7 @@ nb_status render_frames_sample_parallel(nb_context *ctx, const nb_scene *scene, 
         h.request.scene = 0;
         h.request.bounces = first->bounces;
         h.request.contribution_factor = first->contribution_factor;
+        h.request.rr_first = ctx->rr_first;
         h.pass = ServePass{0u, first->sample + done, count, pixels, 0u, 0u, 0u, 0u};
         h.group = ServeGroup{0u, 0u, count * (pixels / 16u), 0u, 1u, {0u, 0u, 0u}};
         ServeStep &st = h.step;
@@ -750,6 +752,53 @@ nb_status nb_envmap_create_cube(nb_context *ctx, const float *texels, uint32_t f
     }
     ++ctx->children;
     *out = m;
+    return NB_OK;
+}
+
+nb_status nb_envmap_create_latlong(nb_context *ctx, const float *texels, uint32_t width, uint32_t height,
+                                   nb_envmap **out) {
+    if (!ctx || !texels || !out || width == 0 || height == 0 || width > 65536 || height > 32768)
+        return set_error(NB_ERR_INVALID_ARGUMENT, "bad lat-long environment arguments");
+    *out = nullptr;
+    NB_TRY(use_device(ctx));
+    // one border texel on every side: the columns wrap across the u seam, the rows repeat the
+    // pole rows, so the lookup's four taps never leave the padded map
+    const uint32_t pw = width + 2, ph = height + 2;
+    std::vector<float4> padded;
+    try {
+        padded.resize(static_cast<size_t>(pw) * ph);
+    } catch (const std::bad_alloc &) {
+        return set_error(NB_ERR_OUT_OF_MEMORY, "host allocation failed");
+    }
+    for (uint32_t j = 0; j < ph; ++j) {
+        const uint32_t sj = j == 0 ? 0u : (j == ph - 1 ? height - 1 : j - 1);
+        for (uint32_t i = 0; i < pw; ++i) {
+            const uint32_t si = i == 0 ? width - 1 : (i == pw - 1 ? 0u : i - 1);
+            const float *t = texels + (static_cast<size_t>(sj) * width + si) * 4;
+            padded[static_cast<size_t>(j) * pw + i] = make_float4(t[0], t[1], t[2], t[3]);
+        }
+    }
+    nb_envmap *m = new nb_envmap();
+    m->ctx = ctx;
+    m->n = 0;
+    m->width = width;
+    m->height = height;
+    hipError_t e = hipMalloc(&m->texels, padded.size() * 16);
+    if (e == hipSuccess) e = hipMemcpy(m->texels, padded.data(), padded.size() * 16, hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        hipFree(m->texels);
+        delete m;
+        return set_error(e == hipErrorOutOfMemory ? NB_ERR_OUT_OF_MEMORY : NB_ERR_HIP, "environment upload: %s",
+                         hipGetErrorString(e));
+    }
+    ++ctx->children;
+    *out = m;
+    return NB_OK;
+}
+
+nb_status nb_render_options_set(nb_context *ctx, const nb_render_options *options) {
+    if (!ctx || !options) return set_error(NB_ERR_INVALID_ARGUMENT, "null argument");
+    ctx->rr_first = options->roulette_first_bounce;
     return NB_OK;
 }
 

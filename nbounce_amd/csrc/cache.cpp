@@ -469,7 +469,7 @@ nb_status nb_cache_training_records(nb_context *ctx, const nb_scene *scene, uint
     const size_t npix = static_cast<size_t>(width) * height;
     CacheFrame f{};
     f.scene = scene->dev;
-    f.env = DevEnv{env->texels, env->n};
+    f.env = dev_env(env);
     f.sobol = ctx->sobol;
     f.sobol_entries = ctx->sobol_entries;
     f.sobol_stride = ctx->sobol_stride;
@@ -510,7 +510,7 @@ nb_status nb_render_cached(nb_context *ctx, const nb_scene *scene, nb_target *ta
     if (g->sample == 0) return set_error(NB_ERR_INVALID_ARGUMENT, "samples are 1-based");
     C_HIP(hipSetDevice(ctx->device));
     if (ctx->render_timing) C_HIP(hipEventRecord(ctx->ev_start, ctx->stream));
-    C_TRY(cache_render_frame(ctx, scene->dev, DevEnv{env->texels, env->n}, target->pixels, target->width,
+    C_TRY(cache_render_frame(ctx, scene->dev, dev_env(env), target->pixels, target->width,
                              target->height, camera->clip_to_world, *g, c, train_paths, lr, loss));
     if (ctx->render_timing) {
         C_HIP(hipEventRecord(ctx->ev_stop, ctx->stream));

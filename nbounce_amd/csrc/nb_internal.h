@@ -76,8 +76,12 @@ struct DevScene {
 };
 
 struct DevEnv {
-    const float4 *texels;  // 6 faces of (n+2)^2 texels, one texel of seamless border around each face
+    // cube (n > 0): 6 faces of (n+2)^2 texels, one texel of seamless border around each face;
+    // lat-long (n == 0): height+2 rows of width+2 texels, the border columns wrapped from the
+    // opposite side and the border rows repeating the edge rows
+    const float4 *texels;
     uint32_t n;
+    uint32_t width, height;   // lat-long size
 };
 
 struct RenderParams {
@@ -103,6 +107,7 @@ struct RenderParams {
     uint32_t grab_units;     // work units per grab before that
     const uint32_t *unit_order;  // position -> work unit, a permutation within each XCD band, or null
     uint32_t *unit_cost;     // rays traced per work unit, added as pixels finish, or null
+    uint32_t rr_first;       // first bounce with Russian roulette (the reference: 0)
     float4 *sample_out;      // or null; else (n_frames 1) each pixel's sample goes here, the target
                              // is neither read nor written
 };
@@ -121,7 +126,8 @@ struct ServeRequestDev {
     uint32_t scene;          // scene slot
     uint32_t bounces;
     float contribution_factor;
-    uint32_t pad[3];
+    uint32_t rr_first;       // first bounce with Russian roulette (the reference: 0)
+    uint32_t pad[2];
 };
 
 // The samples one request renders in one step: samples first_sample .. first_sample + count - 1
@@ -185,6 +191,7 @@ struct nb_context {
     uint32_t sobol_entries = 0, sobol_stride = 17;
     unsigned long long *counters = nullptr;   // kCounterSlots x kCounterWords
     bool counting = true;                     // nb_counters_enable
+    uint32_t rr_first = 0;                    // nb_render_options_set: first bounce with roulette
     uint32_t *work = nullptr;       // tile counters of the persistent render kernel
     uint32_t *hits_scratch = nullptr;   // device address of nb_primary_hits_noalloc's scratch (not owned)
     // Sample-parallel nb_render_frames (small targets): the serving kernels' step description
@@ -259,8 +266,18 @@ nb_status unit_order_update(nb_target *t, const RenderParams &p, uint32_t n_fram
 struct nb_envmap {
     nb_context *ctx = nullptr;
     float4 *texels = nullptr;
-    uint32_t n = 0;
+    uint32_t n = 0;                     // cube face size; 0 for a lat-long map
+    uint32_t width = 0, height = 0;     // lat-long size
 };
+
+inline nb::DevEnv dev_env(const nb_envmap *e) {
+    return e ? nb::DevEnv{e->texels, e->n, e->width, e->height} : nb::DevEnv{nullptr, 0u, 0u, 0u};
+}
+
+// Device bytes of an environment map's padded texels.
+inline uint64_t envmap_device_bytes(const nb_envmap *e) {
+    return e->n ? 6ull * (e->n + 2) * (e->n + 2) * 16 : static_cast<uint64_t>(e->width + 2) * (e->height + 2) * 16;
+}
 
 
 namespace nb {

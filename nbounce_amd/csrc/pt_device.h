@@ -359,9 +359,65 @@ __device__ __forceinline__ void trace(const DevScene &S, const Ray &world, Hit &
     hit = ts.hit;
 }
 
+// atan on binary32 by Cephes' atanf (S. L. Moshier): |x| above tan(3pi/8) maps to pi/2 +
+// atan(-1/|x|), above tan(pi/8) to pi/4 + atan((|x|-1)/(|x|+1)), then z + z^3 P(z^2) with
+// Cephes' degree-3 P; the sign of x is restored at the end.
+__device__ __forceinline__ float atan_f(float x) {
+    const float ax = fabsf(x);
+    float y0 = 0.0f, z = ax;
+    if (ax > 2.414213562373095f) {
+        y0 = 1.5707963267948966f;
+        z = -1.0f / ax;
+    } else if (ax > 0.4142135623730950f) {
+        y0 = 0.7853981633974483f;
+        z = (ax - 1.0f) / (ax + 1.0f);
+    }
+    const float z2 = z * z;
+    const float q = (((8.05374449538e-2f * z2 - 1.38776856032e-1f) * z2 + 1.99777106478e-1f) * z2 - 3.33329491539e-1f) * z2;
+    const float r = y0 + (q * z + z);
+    return x < 0.0f ? -r : r;
+}
+
+// atan2 on binary32 from atan_f: x = 0 gives +-pi/2 (0 when y is 0 too); x < 0 adds pi for
+// y >= 0 (so both signed zeros of y give +pi) and subtracts it for y < 0.
+__device__ __forceinline__ float atan2_f(float y, float x) {
+    if (x == 0.0f) return y > 0.0f ? 1.5707963267948966f : (y < 0.0f ? -1.5707963267948966f : 0.0f);
+    const float a = atan_f(y / x);
+    if (x > 0.0f) return a;
+    return y >= 0.0f ? a + 3.14159265358979323846f : a - 3.14159265358979323846f;
+}
+
+// Equirectangular lookup (an environment form the reference does not have; it reads only the
+// cube): y is up, u = atan2(d.z, d.x) / 2pi + 1/2 (+x at u = 1/2, +z at 3/4, -z at 1/4) and
+// v = atan2(|d.xz|, d.y) / pi (+y at the top row), bilinear at LOD 0 over the padded map, so
+// the four taps wrap across the u seam and repeat the pole rows.
+__device__ F3 latlong_sample(const DevEnv &E, F3 d) {
+    const float r = sqrtf(d.x * d.x + d.z * d.z);
+    const float u = atan2_f(d.z, d.x) * 0.15915494309189535f + 0.5f;
+    const float v = atan2_f(r, d.y) * 0.31830988618379067f;
+    const float w = static_cast<float>(E.width), h = static_cast<float>(E.height);
+    const float x = u * w - 0.5f, y = v * h - 0.5f;
+    float fx = floorf(x), fy = floorf(y);
+    if (!(fx >= -1.0f)) fx = -1.0f;
+    if (fx > w - 1.0f) fx = w - 1.0f;
+    if (!(fy >= -1.0f)) fy = -1.0f;
+    if (fy > h - 1.0f) fy = h - 1.0f;
+    const float a = x - fx, b = y - fy;
+    const uint32_t pw = E.width + 2;
+    const int i0 = static_cast<int>(fx) + 1, j0 = static_cast<int>(fy) + 1;
+    const float4 *base = E.texels + static_cast<size_t>(j0) * pw + i0;
+    const float4 t00 = base[0], t10 = base[1], t01 = base[pw], t11 = base[pw + 1];
+    const float w00 = (1.0f - a) * (1.0f - b), w10 = a * (1.0f - b), w01 = (1.0f - a) * b, w11 = a * b;
+    return f3(((w00 * t00.x + w10 * t10.x) + w01 * t01.x) + w11 * t11.x,
+              ((w00 * t00.y + w10 * t10.y) + w01 * t01.y) + w11 * t11.y,
+              ((w00 * t00.z + w10 * t10.z) + w01 * t01.z) + w11 * t11.z);
+}
+
 // Bilinear cube lookup at LOD 0 with Vulkan face selection; faces carry a one-texel border
-// holding the seamless-edge neighbours, so the four taps never leave the padded face.
+// holding the seamless-edge neighbours, so the four taps never leave the padded face.  A map of
+// face size 0 is a lat-long map.
 __device__ F3 env_sample(const DevEnv &E, F3 d) {
+    if (E.n == 0) return latlong_sample(E, d);
     float ax = fabsf(d.x), ay = fabsf(d.y), az = fabsf(d.z);
     int f;
     float ma, sc, tc;
@@ -464,6 +520,7 @@ struct PathConst {
     SobolView sobol;
     uint32_t sample, bounces;
     float contribution_factor;
+    uint32_t rr_first;       // first bounce with Russian roulette (the reference: 0)
     float4 shift;
 };
 
@@ -568,14 +625,17 @@ __device__ StepResult shade(const PathConst &pc, uint32_t bounce, const Hit &h, 
         F3 diff = f3((om * albedo.x) * resp, (om * albedo.y) * resp, (om * albedo.z) * resp);
         thr = mul(thr, divs(diff, p_diff));
     }
-    // Russian roulette, pathtracing.wgsl:227-236
-    float p_cont = fminf(1.0f - pow8(static_cast<float>(bounce) / static_cast<float>(pc.bounces)), 1.0f);
-    p_cont *= fminf(luminance(thr) * pc.contribution_factor, 1.0f);
-    if (sob0.z < p_cont) {
-        thr = divs(thr, p_cont);
-    } else {
-        out = f3(0.0f, 0.0f, 0.0f);
-        return kDone;
+    // Russian roulette, pathtracing.wgsl:227-236, from bounce rr_first on (the reference: 0;
+    // before it every path continues with its throughput as it is)
+    if (bounce >= pc.rr_first) {
+        float p_cont = fminf(1.0f - pow8(static_cast<float>(bounce) / static_cast<float>(pc.bounces)), 1.0f);
+        p_cont *= fminf(luminance(thr) * pc.contribution_factor, 1.0f);
+        if (sob0.z < p_cont) {
+            thr = divs(thr, p_cont);
+        } else {
+            out = f3(0.0f, 0.0f, 0.0f);
+            return kDone;
+        }
     }
     ray.o = pos;
     ray.d = wi;

@@ -179,6 +179,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5))) void se
                     pc.sample = P->first_sample + k;
                     pc.bounces = R->bounces;
                     pc.contribution_factor = R->contribution_factor;
+                    pc.rr_first = R->rr_first;
                     const float4 jit = sobol_fetch(pc.sobol, pc.sample * S.sobol_stride, hash4f(x, y, x, y));
                     ray = camera_ray(R->c2w, width, R->height, x, y, jit.x, jit.y);
                     thr = f3(1.0f, 1.0f, 1.0f);

@@ -190,6 +190,7 @@ nb_status admit(nb_server *s) {
         d.scene = r->req.scene;
         d.bounces = r->req.bounces;
         d.contribution_factor = r->req.contribution_factor;
+        d.rr_first = 0;   // served requests use the reference's roulette
     }
     return NB_OK;
 }
@@ -267,7 +268,7 @@ nb_status nb_server_add_scene(nb_server *s, const nb_scene *scene, const nb_envm
     // a scene or environment shared by several resident scenes is charged once
     uint64_t add = 0;
     if (!s->counted.count(scene)) add += scene->bytes;
-    if (!s->counted.count(env)) add += 6ull * (env->n + 2) * (env->n + 2) * 16;
+    if (!s->counted.count(env)) add += envmap_device_bytes(env);
     if (s->in_use + add > s->budget)
         return set_error(NB_ERR_BUDGET, "scene %u needs %llu B, %llu B of the HBM budget are left", s->n_scenes,
                          static_cast<unsigned long long>(add), static_cast<unsigned long long>(s->budget - s->in_use));
@@ -279,7 +280,7 @@ nb_status nb_server_add_scene(nb_server *s, const nb_scene *scene, const nb_envm
     }
     s->resident += add;
     s->in_use += add;
-    s->h_scenes[s->n_scenes] = ServeSceneDev{scene->dev, DevEnv{env->texels, env->n}};
+    s->h_scenes[s->n_scenes] = ServeSceneDev{scene->dev, dev_env(env)};
     SRV_HIP(hipSetDevice(s->ctx->device));
     SRV_HIP(hipMemcpy(s->d_scenes, s->h_scenes, (s->n_scenes + 1) * sizeof(ServeSceneDev), hipMemcpyHostToDevice));
     *index = s->n_scenes++;

@@ -40,6 +40,13 @@ class Context:
     def reset_counters(self):
         check(lib.nb_counters_reset(self.handle))
 
+    def set_render_options(self, roulette_first_bounce=0):
+        """Native options of nb_render / nb_render_frames on this context (no counterpart in the
+        reference): Russian roulette from bounce `roulette_first_bounce` on (0: the reference's
+        rule, pathtracing.wgsl:227-236)."""
+        opts = np.array([roulette_first_bounce], dtype=np.uint32)
+        check(lib.nb_render_options_set(self.handle, ptr(opts)))
+
     def enable_render_timing(self, on=True):
         """Per-render HIP events for last_render_ms (off by default: they cost a frame stream
         about 0.5 %)."""
@@ -229,6 +236,22 @@ class EnvMap:
         check(lib.nb_envmap_create_cube(ctx.handle, ptr(faces), faces.shape[1], ctypes.byref(h)))
         self.handle = h
         self.faces = faces
+
+    @classmethod
+    def latlong(cls, ctx, texels):
+        """An equirectangular (height, width, 4) RGBA32F map, row 0 at the top (+y); no counterpart
+        in the reference, which samples only the cube.  `faces` is None and `texels` holds it."""
+        texels = np.ascontiguousarray(texels, dtype=np.float32)
+        if texels.ndim != 3 or texels.shape[2] != 4:
+            raise ValueError("texels must be (height, width, 4) RGBA32F")
+        self = cls.__new__(cls)
+        h = ctypes.c_void_p()
+        check(lib.nb_envmap_create_latlong(ctx.handle, ptr(texels), texels.shape[1], texels.shape[0],
+                                           ctypes.byref(h)))
+        self.handle = h
+        self.faces = None
+        self.texels = texels
+        return self
 
     @classmethod
     def load(cls, ctx, path):

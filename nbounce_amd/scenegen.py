@@ -333,6 +333,24 @@ def sky_cube(n=128):
     return out
 
 
+def latlong_directions(width, height):
+    """Unit directions of the texel centres of an equirectangular map (height, width, 3): y up,
+    u = atan2(z, x) / 2pi + 1/2, v = polar angle / pi, row 0 at +y."""
+    u = (np.arange(width) + 0.5) / width
+    v = (np.arange(height) + 0.5) / height
+    phi = (u - 0.5) * 2.0 * np.pi
+    theta = v * np.pi
+    st, ct = np.sin(theta)[:, None], np.cos(theta)[:, None]
+    return np.stack([st * np.cos(phi)[None, :], np.broadcast_to(ct, (height, width)), st * np.sin(phi)[None, :]], -1)
+
+
+def sky_latlong(width=512, height=256):
+    """The analytic sky baked into an RGBA32F equirectangular map (height, width, 4)."""
+    out = np.ones((height, width, 4), dtype=np.float32)
+    out[..., :3] = sky_gradient(latlong_directions(width, height))
+    return out
+
+
 def constant_cube(n=4, value=1.0):
     out = np.full((6, n, n, 4), value, dtype=np.float32)
     out[..., 3] = 1.0
@@ -346,6 +364,9 @@ SMALL = dict(width=256, height=256, spp=16, bounces=8, contribution_factor=4.0, 
              eye=(5.0, 5.0, 9.5), target=(5.0, 4.0, 0.0))
 LARGE = dict(width=1920, height=1080, spp=64, bounces=12, contribution_factor=4.0, env_face=256,
              eye=(-20.0, 110.0, -20.0), target=(100.0, 95.0, 100.0))
+# A second view of the same scene from inside the instance cloud (its centre), for comparison
+# with the default view from outside it, where more of the rays reach the sky.
+LARGE_INTERIOR = dict(eye=(100.0, 100.0, 100.0), target=(190.0, 85.0, 150.0))
 
 # Learned-cache workload: the spheres-in-box scene at 1920x1080, one sample per frame for 240 frames
 # while the camera orbits; errors are measured at the orbit's final pose on every 8th pixel.

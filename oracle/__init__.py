@@ -34,7 +34,8 @@ class _Scene(ctypes.Structure):
 
 
 class _Env(ctypes.Structure):
-    _fields_ = [("texels", ctypes.c_void_p), ("face_size", ctypes.c_uint32)]
+    _fields_ = [("texels", ctypes.c_void_p), ("face_size", ctypes.c_uint32), ("width", ctypes.c_uint32),
+                ("height", ctypes.c_uint32)]
 
 
 def build():
@@ -50,6 +51,11 @@ def _load():
     lib.orc_sin.argtypes = [ctypes.c_float]
     lib.orc_cos.restype = ctypes.c_float
     lib.orc_cos.argtypes = [ctypes.c_float]
+    lib.orc_atan.restype = ctypes.c_float
+    lib.orc_atan.argtypes = [ctypes.c_float]
+    lib.orc_atan2.restype = ctypes.c_float
+    lib.orc_atan2.argtypes = [ctypes.c_float, ctypes.c_float]
+    lib.orc_set_roulette_first.argtypes = [ctypes.c_uint32]
     lib.orc_intersect_aabb.restype = ctypes.c_float
     lib.orc_intersect_aabb.argtypes = [ctypes.c_void_p] * 4
     lib.orc_sobol_unscrambled.restype = ctypes.c_uint32
@@ -118,6 +124,19 @@ def map4f(x):
     return out
 
 
+def atan(x):
+    return lib.orc_atan(float(x))
+
+
+def atan2(y, x):
+    return lib.orc_atan2(float(y), float(x))
+
+
+def set_roulette_first(bounce):
+    """First bounce with Russian roulette in every later oracle render (0: the reference's rule)."""
+    lib.orc_set_roulette_first(int(bounce))
+
+
 def mat4_inverse(m16):
     m = np.ascontiguousarray(m16, dtype=np.float32)
     out = np.zeros(16, dtype=np.float32)
@@ -179,7 +198,15 @@ class Scene:
 class Env:
     def __init__(self, faces):
         self.faces = np.ascontiguousarray(faces, dtype=np.float32)
-        self.e = _Env(p(self.faces), self.faces.shape[1])
+        self.e = _Env(p(self.faces), self.faces.shape[1], 0, 0)
+
+    @classmethod
+    def latlong(cls, texels):
+        """The lat-long form: (height, width, 4) RGBA32F, row 0 at +y."""
+        self = cls.__new__(cls)
+        self.faces = np.ascontiguousarray(texels, dtype=np.float32)
+        self.e = _Env(p(self.faces), 0, self.faces.shape[1], self.faces.shape[0])
+        return self
 
     @property
     def ref(self):

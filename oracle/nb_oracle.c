@@ -129,6 +129,43 @@ float orc_cos(float x) {
 }
 
 /* ------------------------------------------------------------------------------------------
+ * atan / atan2, for the lat-long environment form (no counterpart in the reference, which
+ * samples only the cube): Cephes atanf (S. Moshier).  |x| > tan(3pi/8) becomes pi/2 +
+ * atan(-1/|x|), |x| > tan(pi/8) becomes pi/4 + atan((|x|-1)/(|x|+1)); then the degree-3
+ * minimax polynomial in z^2 gives z + z^3 P(z^2).  atan2 adds +-pi for x < 0 (+pi when y >= 0,
+ * signed zeros included) and returns +-pi/2 or 0 for x == 0.
+ * ------------------------------------------------------------------------------------------ */
+float orc_atan(float x) {
+    float ax = fabsf(x), base = 0.0f, z = ax;
+    if (ax > 2.414213562373095f) {
+        base = 1.5707963267948966f;
+        z = -1.0f / ax;
+    } else if (ax > 0.4142135623730950f) {
+        base = 0.7853981633974483f;
+        z = (ax - 1.0f) / (ax + 1.0f);
+    }
+    float zz = z * z;
+    float poly = 8.05374449538e-2f;
+    poly = poly * zz - 1.38776856032e-1f;
+    poly = poly * zz + 1.99777106478e-1f;
+    poly = poly * zz - 3.33329491539e-1f;
+    poly = poly * zz;
+    float r = base + (poly * z + z);
+    return x < 0.0f ? -r : r;
+}
+float orc_atan2(float y, float x) {
+    if (x == 0.0f) {
+        if (y > 0.0f) return 1.5707963267948966f;
+        if (y < 0.0f) return -1.5707963267948966f;
+        return 0.0f;
+    }
+    float a = orc_atan(y / x);
+    if (x > 0.0f) return a;
+    if (y >= 0.0f) return a + 3.14159265358979323846f;
+    return a - 3.14159265358979323846f;
+}
+
+/* ------------------------------------------------------------------------------------------
  * Mat4 (column-major, m[c*4 + r]) as glam 0.29 computes it.
  * ------------------------------------------------------------------------------------------ */
 /* Cofactor inverse in glm's arrangement (glam Mat4::inverse); determinant as (x+z)+(y+w),
@@ -884,8 +921,45 @@ static void fetch_texel(const orc_env *env, int f, int i, int j, float out[3]) {
     out[0] = t[0]; out[1] = t[1]; out[2] = t[2];
 }
 
+/* The lat-long form (face_size 0): u = atan2(d.z, d.x)/2pi + 1/2, v = atan2(|d.xz|, d.y)/pi,
+ * bilinear at LOD 0; texel columns wrap across the u seam and rows clamp at the poles. */
+static void latlong_texel(const orc_env *env, int i, int j, float out[3]) {
+    int w = (int)env->width, h = (int)env->height;
+    if (i < 0) i = w - 1;
+    else if (i >= w) i = 0;
+    if (j < 0) j = 0;
+    else if (j >= h) j = h - 1;
+    const float *t = env->texels + ((size_t)j * (size_t)w + (size_t)i) * 4;
+    out[0] = t[0]; out[1] = t[1]; out[2] = t[2];
+}
+static void latlong_sample(const orc_env *env, v3 d, float out[3]) {
+    float r = sqrtf(d.x * d.x + d.z * d.z);
+    float u = orc_atan2(d.z, d.x) * 0.15915494309189535f + 0.5f;
+    float v = orc_atan2(r, d.y) * 0.31830988618379067f;
+    float w = (float)env->width, h = (float)env->height;
+    float x = u * w - 0.5f, y = v * h - 0.5f;
+    float fx = floorf(x), fy = floorf(y);
+    if (!(fx >= -1.0f)) fx = -1.0f;
+    if (fx > w - 1.0f) fx = w - 1.0f;
+    if (!(fy >= -1.0f)) fy = -1.0f;
+    if (fy > h - 1.0f) fy = h - 1.0f;
+    float a = x - fx, b = y - fy;
+    int i0 = (int)fx, j0 = (int)fy;
+    float t00[3], t10[3], t01[3], t11[3];
+    latlong_texel(env, i0, j0, t00);
+    latlong_texel(env, i0 + 1, j0, t10);
+    latlong_texel(env, i0, j0 + 1, t01);
+    latlong_texel(env, i0 + 1, j0 + 1, t11);
+    float w00 = (1.0f - a) * (1.0f - b), w10 = a * (1.0f - b), w01 = (1.0f - a) * b, w11 = a * b;
+    for (int k = 0; k < 3; ++k) out[k] = ((w00 * t00[k] + w10 * t10[k]) + w01 * t01[k]) + w11 * t11[k];
+}
+
 void orc_env_sample(const orc_env *env, const float dir[3], float out[3]) {
     v3 d = ld3(dir);
+    if (env->face_size == 0) {
+        latlong_sample(env, d, out);
+        return;
+    }
     int f = select_face(d);
     float dc[3] = {d.x, d.y, d.z};
     float ma = fabsf(dc[FACE_MA[f]]);
@@ -997,6 +1071,12 @@ static void sobol_fetch(const float *table, uint32_t entries, uint32_t entry, co
     for (int k = 0; k < 4; ++k) out[k] = fract1(s[k] + shift[k]);
 }
 
+/* First bounce with Russian roulette (orc_set_roulette_first): 0 is the reference's rule
+ * (pathtracing.wgsl:227-236); a larger value lets every path continue, throughput unchanged,
+ * through the bounces before it (the library's nb_render_options). */
+static uint32_t g_roulette_first = 0;
+void orc_set_roulette_first(uint32_t bounce) { g_roulette_first = bounce; }
+
 /* sample_rendering_eq, pathtracing.wgsl:161-241 */
 /* Shading points of a path as the learned cache's training records see them (no counterpart in
  * the reference): up to max_rec non-emissive hits, each as 16 floats (position, outgoing
@@ -1075,10 +1155,12 @@ static v3 sample_rendering_eq_rec(const orc_scene *sc, const orc_globals *g, con
             v3 diff = mk3((om * albedo.x) * resp, (om * albedo.y) * resp, (om * albedo.z) * resp);
             thr = mul3(thr, divs3(diff, p_diff));
         }
-        float p_cont = fmin_(1.0f - pow8f((float)bounce / (float)g->bounces), 1.0f);
-        p_cont *= fmin_(luminance(thr) * g->contribution_factor, 1.0f);
-        if (s0[2] < p_cont) thr = divs3(thr, p_cont);
-        else return mk3(0.0f, 0.0f, 0.0f);
+        if (bounce >= g_roulette_first) {   /* the reference: from bounce 0 */
+            float p_cont = fmin_(1.0f - pow8f((float)bounce / (float)g->bounces), 1.0f);
+            p_cont *= fmin_(luminance(thr) * g->contribution_factor, 1.0f);
+            if (s0[2] < p_cont) thr = divs3(thr, p_cont);
+            else return mk3(0.0f, 0.0f, 0.0f);
+        }
         ray.o = hit.position; ray.d = wi; ray.inv = div3(mk3(1.0f, 1.0f, 1.0f), wi);
     }
     return mk3(0.0f, 0.0f, 0.0f);

@@ -12,6 +12,9 @@
  * (the test that checks it):
  *   orc_hash4u / orc_map4f    common.wgsl:7-9,27-41     inverse round trip, closed-form map values
  *   orc_sin / orc_cos         (own Cody-Waite + Cephes)  float64 numpy sin/cos, <= 2 ulp
+ *   orc_atan / orc_atan2      (own, Cephes atanf; the lat-long environment form, no reference)
+ *                                                        float64 numpy arctan2 over all quadrants,
+ *                                                        axes and both reduction branches
  *   orc_mat4_inverse          glam Mat4::inverse         M*M^-1 = I, numpy.linalg.inv;
  *                                                        bit order vs glam 0.29: parity unpinned
  *   orc_camera                camera.rs:121-142          numpy restatement of look_at/perspective
@@ -29,7 +32,12 @@
  *                             Owen scrambling pinned by (0,m,2)-net stratification;
  *                             bits vs sobol_burley 0.5.0: parity unpinned
  *   orc_env_sample            texture.rs:126-141, pathtracing.wgsl:168 (Vulkan cube rules)
- *                                                        constant / linear fields, face centres
+ *                                                        constant / linear fields, face centres;
+ *                             lat-long form (no reference): constant field, linear-in-u rows,
+ *                             one-hot texels at +x / +z / -z / +y, seam continuity, the baked
+ *                             sky against the cube bake of the same sky
+ *   orc_set_roulette_first    (no reference: nb_render_options) forced roulette numbers give a
+ *                             path of exactly first + 1 rays; 0 leaves the reference's rule
  *   orc_render                pathtracing.wgsl:27-272    white-furnace / closed-form single-bounce
  *                                                        expectations, RR unbiasedness; the
  *                                                        multi-bounce loop by hand-derived path
@@ -75,8 +83,10 @@ typedef struct {
 } orc_scene;
 
 typedef struct {
-    const float *texels;   /* 6 faces, each face_size x face_size RGBA32F, Vulkan layer order +X,-X,+Y,-Y,+Z,-Z */
+    const float *texels;   /* cube: 6 faces, each face_size x face_size RGBA32F, Vulkan layer order +X,-X,+Y,-Y,+Z,-Z;
+                              lat-long (face_size 0): height rows of width RGBA32F, row 0 at +y */
     uint32_t face_size;
+    uint32_t width, height;
 } orc_env;
 
 typedef struct {
@@ -95,6 +105,8 @@ void orc_map4f(const uint32_t x[4], float out[4]);
 /* ---- math ---- */
 float orc_sin(float x);
 float orc_cos(float x);
+float orc_atan(float x);
+float orc_atan2(float y, float x);
 void orc_mat4_inverse(const float m[16], float out[16]);
 void orc_mat4_mul(const float a[16], const float b[16], float out[16]);
 void orc_camera_compute(const float eye[3], const float target[3], const float up[3],
@@ -137,6 +149,9 @@ void orc_sobol_table(uint32_t rows, uint32_t dims, uint32_t seed, float *out);
 
 /* ---- environment ---- */
 void orc_env_sample(const orc_env *env, const float dir[3], float out[3]);
+
+/* ---- options (the library's nb_render_options; not in the reference) ---- */
+void orc_set_roulette_first(uint32_t bounce);
 
 /* ---- integrator ---- */
 /* Renders one sample per pixel into `rgba` (width*height*4, row-major) exactly as one dispatch of

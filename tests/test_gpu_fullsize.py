@@ -88,6 +88,28 @@ def test_fullsize_band_matches_oracle(full):
     assert err.max() <= REL_TOL
 
 
+def test_fullsize_interior_view_band_matches_oracle(full):
+    """The same scene from inside the instance cloud (more hits, longer BLAS descents)."""
+    nb, ctx, sb, env, cam, osc, faces, P = full
+    from nbounce_amd import scenegen
+    V = scenegen.LARGE_INTERIOR
+    icam = nb.CameraController(world_position=V["eye"], target=V["target"], aspect_ratio=WIDTH / HEIGHT)
+    print("case: 1920x1080 instanced scene from inside the cloud, 1 frame, rows 532..548 against the oracle",
+          flush=True)
+    ctx.reset_counters()
+    got = _render(_tracer(nb, ctx, sb, env, icam, P), 1)
+    c = ctx.counters()
+    y0, y1 = HEIGHT // 2 - 8, HEIGHT // 2 + 8
+    g = oracle.globals_(1, 1.0, P["bounces"], P["contribution_factor"])
+    img, _ = oracle.render(osc, icam.data(), g, W.table(), 17, oracle.Env(faces), WIDTH, HEIGHT,
+                           rect=(0, y0, WIDTH, y1), threads=8)
+    band_got, band_ref = got[y0:y1], img[y0:y1]
+    same = (band_got.view(np.uint32) == band_ref.view(np.uint32)).all(-1).mean()
+    err = W.rel_err(band_got, band_ref)
+    print(f"hits per ray {c['hits'] / c['rays']:.3f}, max rel err {err.max():.3e}, bit-identical pixels {same:.6f}")
+    assert err.max() <= REL_TOL
+
+
 def test_fullsize_persistent_equals_tile_kernel(full):
     nb, ctx, sb, env, cam, osc, faces, P = full
     print("case: 1920x1080, 3 frames, persistent vs one-tile-per-wave kernel", flush=True)
