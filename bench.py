@@ -211,17 +211,24 @@ def view_rate(nb, ctx, sb, env, P, eye, target, frames=32):
                                    "(inside the instance cloud)"}}
 
 
-def small_workload_rate(frames=16):
+def small_workload_rate(frames=16, native=False):
     """The small workload (64 icospheres in a box, 256x256, 8 bounces) at `frames` samples per
     pixel in one nb_render_frames call (sample-parallel): Mrays/s over an untimed counted pass and
-    a timed one of the same frames."""
+    a timed one of the same frames.  native: as the configuration is written (Russian roulette
+    from bounce 3, the sky as a 512x256 RGBA32F lat-long map) instead of the reference's rules
+    (roulette from bounce 0, the sky baked into a cube)."""
     import nbounce_amd as nb
     from nbounce_amd import scenegen
     P, desc, scene, arrays, _ = build_workload("small")
     blas, tlas, inst = arrays
     ctx = nb.Context(0)
     sb = nb.SceneBuffers(ctx, blas, tlas, inst, scene.vertices, scene.indices)
-    env = nb.EnvMap(ctx, scenegen.sky_cube(P["env_face"]))
+    if native:
+        env = nb.EnvMap.latlong(ctx, scenegen.sky_latlong(512, 256))
+        ctx.set_render_options(roulette_first_bounce=3)
+        desc += ", Russian roulette from bounce 3, 512x256 lat-long sky"
+    else:
+        env = nb.EnvMap(ctx, scenegen.sky_cube(P["env_face"]))
     cam = nb.CameraController(world_position=P["eye"], target=P["target"], aspect_ratio=1.0)
     pt = nb.Pathtracer(ctx, sb, cam, env, P["width"], P["height"], resolution_factor=1.0)
     pt.globals.bounces, pt.globals.contribution_factor = P["bounces"], P["contribution_factor"]
@@ -581,16 +588,22 @@ def main():
     secondary = None
     if not args.no_secondary and args.workload == "large":
         interior = view_rate(nb, ctx, sb, env, P, scenegen.LARGE_INTERIOR["eye"], scenegen.LARGE_INTERIOR["target"])
-        del pt, sb, env
+        ll_env = nb.EnvMap.latlong(ctx, scenegen.sky_latlong(1024, 512))
+        latlong = view_rate(nb, ctx, sb, ll_env, P, P["eye"], P["target"])
+        latlong["config"]["workload"] = (f"the default workload with the sky as a 1024x512 RGBA32F lat-long map "
+                                         f"instead of the {P['env_face']}^2 cube, 32 frames of 1 spp")
+        del pt, sb, env, ll_env
         ctx.close()
         secondary = {
             "interior_view": interior,
+            "latlong_env": latlong,
             "plain_one_call": {
                 "value": round(rays / wall_one_call / 1e6, 3), "unit": "Mrays/s",
                 "ms_per_image": round(wall_one_call * 1e3, 3),
                 "config": {"workload": f"the same {args.steps} frames of the same workload in one nb_render_frames "
                                        "call (the same image bit for bit)"}},
             "small": small_workload_rate(16),
+            "small_as_written": small_workload_rate(16, native=True),
             "serving": dict(serve_streams([20.0], 100, args.max_paths)[0], config=SERVE_CONFIG),
         }
         c4 = cache_equal_error(1 << 16, 1e-3)
