@@ -211,6 +211,51 @@ def view_rate(nb, ctx, sb, env, P, eye, target, frames=32):
                                    "(inside the instance cloud)"}}
 
 
+def moving_camera_rate(nb, ctx, sb, env, P, frames=32):
+    """Mrays/s of 1-spp nb_render calls at 1920x1080 while the camera moves every frame (each move
+    invalidates the accumulation, as CameraController updates do), pipelined and, for comparison,
+    one launch after another (NB_PIPELINE=0).  An untimed counted pass gives the rays."""
+    def views():
+        for k in range(frames):
+            eye = (P["eye"][0] + 0.25 * k, P["eye"][1] + 0.1 * k, P["eye"][2] + 0.15 * k)
+            yield nb.CameraController(world_position=eye, target=P["target"], aspect_ratio=P["width"] / P["height"])
+
+    def run(pt):
+        for cam in views():
+            pt.update(cam)
+            pt.dispatch()
+
+    pt = nb.Pathtracer(ctx, sb, next(views()), env, P["width"], P["height"], resolution_factor=1.0)
+    pt.globals.bounces = P["bounces"]
+    pt.globals.contribution_factor = P["contribution_factor"]
+    ctx.enable_counters(True)
+    ctx.reset_counters()
+    run(pt)
+    ctx.synchronize()
+    rays = ctx.counters()["rays"]
+    ctx.enable_counters(False)
+    out = {}
+    for mode in ("1", "0"):
+        old = os.environ.get("NB_PIPELINE")
+        os.environ["NB_PIPELINE"] = mode
+        try:
+            ctx.synchronize()
+            t0 = time.perf_counter()
+            run(pt)
+            ctx.synchronize()
+            out[mode] = time.perf_counter() - t0
+        finally:
+            if old is None:
+                del os.environ["NB_PIPELINE"]
+            else:
+                os.environ["NB_PIPELINE"] = old
+    ctx.enable_counters(True)
+    return {"value": round(rays / out["1"] / 1e6, 3), "unit": "Mrays/s", "ms_per_frame": round(out["1"] * 1e3 / frames, 4),
+            "one_launch_after_another": round(rays / out["0"] / 1e6, 3),
+            "config": {"workload": f"the default workload, {frames} frames of 1 spp, the camera moving (and the "
+                                   "accumulation restarting) every frame"}}
+
+
 def small_workload_rate(frames=16, native=False):
     """The small workload (64 icospheres in a box, 256x256, 8 bounces) at `frames` samples per
     pixel in one nb_render_frames call (sample-parallel): Mrays/s over an untimed counted pass and
@@ -588,6 +633,7 @@ def main():
     secondary = None
     if not args.no_secondary and args.workload == "large":
         interior = view_rate(nb, ctx, sb, env, P, scenegen.LARGE_INTERIOR["eye"], scenegen.LARGE_INTERIOR["target"])
+        moving = moving_camera_rate(nb, ctx, sb, env, P)
         ll_env = nb.EnvMap.latlong(ctx, scenegen.sky_latlong(1024, 512))
         latlong = view_rate(nb, ctx, sb, ll_env, P, P["eye"], P["target"])
         latlong["config"]["workload"] = (f"the default workload with the sky as a 1024x512 RGBA32F lat-long map "
@@ -596,6 +642,7 @@ def main():
         ctx.close()
         secondary = {
             "interior_view": interior,
+            "moving_camera": moving,
             "latlong_env": latlong,
             "plain_one_call": {
                 "value": round(rays / wall_one_call / 1e6, 3), "unit": "Mrays/s",

@@ -456,10 +456,11 @@ nb_status render_frames_sample_parallel(nb_context *ctx, const nb_scene *scene, 
 // accumulation step in frame order.  A path kernel waits for the context stream as it was just
 // before the previous frame's mix was enqueued: that includes every earlier mix (the last reader
 // of the sample buffer it writes), not the previous frame's paths, so the two frames' paths run
-// side by side.  When work on the context stream since then changed the work-unit order or reset
-// the counters, it waits for all of the context stream instead.  The image is the same bit for
-// bit (the mix is the same binary32 expression).  NB_PIPELINE=0 renders on the context stream,
-// one launch after another.
+// side by side.  After a counter reset it waits for all of the context stream instead.  The
+// work-unit order's cost halving and rebuilds on the context stream may overlap a path kernel
+// (order.hip: atomic costs, two order buffers).  The image is the same bit for bit (the mix is
+// the same binary32 expression).  NB_PIPELINE=0 renders on the context stream, one launch after
+// another.
 bool pipeline_enabled() {
     const char *e = std::getenv("NB_PIPELINE");
     return !(e && std::strcmp(e, "0") == 0);
@@ -493,14 +494,17 @@ nb_status pipe_setup(nb_context *ctx, nb_target *t) {
 nb_status render_pipelined(nb_context *ctx, const nb_scene *scene, nb_target *t, const nb_camera *cam,
                            const nb_globals *g, RenderParams &p) {
     NB_TRY(pipe_setup(ctx, t));
-    NB_TRY(unit_order_prepare(t, scene, cam, g, p, ctx->stream));
+    // the gate this frame's path kernel waits for: the one recorded before the previous mix, or,
+    // after a counter reset (or on the first frame), all of the context stream
+    const bool refresh = !ctx->pipe_gate_valid || ctx->pipe_fence;
+    NB_TRY(unit_order_prepare(t, scene, cam, g, p, ctx->stream, refresh ? ctx->order_sorts : ctx->pipe_gate_sorts));
     const uint32_t slot = static_cast<uint32_t>(ctx->pipe_seq & 1u);
     hipStream_t side = ctx->pipe_stream[slot];
-    if (!ctx->pipe_gate_valid || ctx->pipe_fence || t->order_dirty) {
+    if (refresh) {
         NB_HIP(hipEventRecord(ctx->pipe_gate, ctx->stream));
+        ctx->pipe_gate_sorts = ctx->order_sorts;
         ctx->pipe_gate_valid = true;
         ctx->pipe_fence = false;
-        t->order_dirty = false;
     }
     NB_HIP(hipStreamWaitEvent(side, ctx->pipe_gate, 0));
     p.sample_out = t->samples[slot];
@@ -513,6 +517,7 @@ nb_status render_pipelined(nb_context *ctx, const nb_scene *scene, nb_target *t,
     }
     NB_HIP(hipEventRecord(ctx->pipe_done, side));
     NB_HIP(hipEventRecord(ctx->pipe_gate, ctx->stream));   // before this frame's mix
+    ctx->pipe_gate_sorts = ctx->order_sorts;
     NB_HIP(hipStreamWaitEvent(ctx->stream, ctx->pipe_done, 0));
     NB_TRY(launch_mix_samples(t->pixels, p.sample_out, t->width * t->height, p.first_weight, ctx->stream));
     ++ctx->pipe_seq;
@@ -897,7 +902,7 @@ nb_status nb_render(nb_context *ctx, const nb_scene *scene, nb_target *target, c
     NB_TRY(use_device(ctx));
     RenderParams p = make_params(ctx, scene, target, camera, globals, env);
     if (pipeline_enabled()) return render_pipelined(ctx, scene, target, camera, globals, p);
-    NB_TRY(unit_order_prepare(target, scene, camera, globals, p, ctx->stream));
+    NB_TRY(unit_order_prepare(target, scene, camera, globals, p, ctx->stream, UINT64_MAX));
     NB_TRY(timed_frames(ctx, p));
     return unit_order_update(target, p, 1u, ctx->stream);
 }
@@ -921,7 +926,7 @@ nb_status nb_render_frames(nb_context *ctx, const nb_scene *scene, nb_target *ta
     RenderParams p = make_params(ctx, scene, target, camera, first, env);
     p.n_frames = n_frames;
     p.weights_from_sample = 1;
-    NB_TRY(unit_order_prepare(target, scene, camera, first, p, ctx->stream));
+    NB_TRY(unit_order_prepare(target, scene, camera, first, p, ctx->stream, UINT64_MAX));
     NB_TRY(timed_frames(ctx, p));
     return unit_order_update(target, p, n_frames, ctx->stream);
 }

@@ -207,9 +207,14 @@ struct nb_context {
     hipEvent_t pipe_done = nullptr;   // end of the last path kernel
     uint64_t pipe_seq = 0;            // pipelined frames so far (the side stream alternates)
     bool pipe_gate_valid = false;
-    // Set when work on `stream` since the last mix changed what a path kernel reads or adds to
-    // (the counters): the next path kernel then waits for all of `stream`.
+    // Set when work on `stream` since the last mix reset what a path kernel adds to (the
+    // counters): the next path kernel then waits for all of `stream`.
     bool pipe_fence = false;
+    // Work-unit order sorts enqueued on `stream` so far (order.hip), and how many of them the
+    // recorded pipe_gate follows: a pipelined path kernel may read an order only once its gate
+    // follows the sort that wrote it.
+    uint64_t order_sorts = 0;
+    uint64_t pipe_gate_sorts = 0;
     // Scenes, targets and environments keep their context alive: nb_context_destroy only marks
     // it, and the last child's destroy releases it.
     uint32_t children = 0;
@@ -230,7 +235,14 @@ struct nb_target {
     // Cost-ordered work units of the persistent kernel (order.hip): rays traced per 16-pixel unit
     // over the frames since the costs were reset, and the order built from them.
     uint32_t n_units = 0;
-    uint32_t *unit_cost = nullptr, *unit_order = nullptr, *sort_vals = nullptr;
+    uint32_t *unit_cost = nullptr, *sort_vals = nullptr;
+    // Two order buffers: a rebuild writes the one no launch in flight may read, and the target
+    // switches to it (order_cur) once the launches that read orders can see the sort.
+    uint32_t *unit_order[2] = {nullptr, nullptr};
+    uint32_t order_cur = 0;
+    bool order_pending = false;     // unit_order[order_cur ^ 1] holds a newer order
+    uint64_t pending_sort = 0;      // the context's sort number that wrote it
+    bool rebuild_wanted = false;    // a rebuild fell due while one was pending
     uint8_t *sort_keys = nullptr;   // 2 x n_units
     int *sort_offsets = nullptr;    // 9 band boundaries
     void *sort_tmp = nullptr;
@@ -242,8 +254,6 @@ struct nb_target {
     nb_camera cost_camera{};
     uint32_t cost_bounces = 0;
     float cost_contribution = 0.0f;
-    // set whenever order.hip enqueues a kernel that rewrites unit_cost or unit_order
-    bool order_dirty = false;
     // Pipelined nb_render: the two frames' sample buffers (RGBA32F per pixel), at first use.
     float4 *samples[2] = {nullptr, nullptr};
 };
@@ -256,10 +266,16 @@ namespace nb {
 // null with NB_UNIT_ORDER=0).  unit_order_update, after the launch of n frames, rebuilds the order
 // from the costs once the view has held for 2, 4, 8, ... frames.  The order changes only which
 // wave renders a pixel when, never a pixel's samples.
+// Both run on the context stream `s` and may overlap a pipelined path kernel of the same target:
+// the costs are halved and read with atomics (a path kernel adds to them as its pixels finish),
+// and a rebuild writes the order buffer no launch in flight reads.  `visible_sorts` is the number
+// of the context's sorts the coming launch follows (all of them for a launch on `s`, the gate's
+// count for a pipelined one); the rebuilt order is used from the first launch that follows its
+// sort.
 nb_status unit_order_alloc(nb_target *t);
 void unit_order_free(nb_target *t);
 nb_status unit_order_prepare(nb_target *t, const nb_scene *scene, const nb_camera *cam, const nb_globals *g,
-                             RenderParams &p, hipStream_t s);
+                             RenderParams &p, hipStream_t s, uint64_t visible_sorts);
 nb_status unit_order_update(nb_target *t, const RenderParams &p, uint32_t n_frames, hipStream_t s);
 }  // namespace nb
 

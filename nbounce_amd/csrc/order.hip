@@ -21,10 +21,14 @@ namespace {
 
 constexpr uint32_t kBands = 8;   // one per XCD, as in make_params / render_pt_kernel
 
-// a[i] >>= shift (32: zero)
+// a[i] >>= shift (32: zero) for the value a[i] held when it was read: one atomic subtraction of
+// the part that goes, so rays that a pipelined path kernel adds meanwhile are kept.
 __global__ void shift_kernel(uint32_t *a, uint32_t n, uint32_t shift) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) a[i] = shift >= 32u ? 0u : (a[i] >> shift);
+    if (i >= n) return;
+    const uint32_t v = __hip_atomic_load(a + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const uint32_t drop = shift >= 32u ? v : v - (v >> shift);
+    if (drop) __hip_atomic_fetch_sub(a + i, drop, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 // Sort key of unit i: its mean rays per frame >> shift, at most 255; value: the unit id.
@@ -32,7 +36,8 @@ __global__ void order_keys_kernel(const uint32_t *cost, uint32_t n, uint32_t fra
                                   uint8_t *keys, uint32_t *vals) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    keys[i] = static_cast<uint8_t>(min((cost[i] / frames) >> shift, 255u));
+    const uint32_t c = __hip_atomic_load(cost + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    keys[i] = static_cast<uint8_t>(min((c / frames) >> shift, 255u));
     vals[i] = i;
 }
 
@@ -80,7 +85,8 @@ nb_status unit_order_alloc(nb_target *t) {
     t->n_units = n;
     t->sort_tmp_bytes = tmp;
     NB_ORDER_HIP(hipMalloc(&t->unit_cost, n * sizeof(uint32_t)));
-    NB_ORDER_HIP(hipMalloc(&t->unit_order, n * sizeof(uint32_t)));
+    NB_ORDER_HIP(hipMalloc(&t->unit_order[0], n * sizeof(uint32_t)));
+    NB_ORDER_HIP(hipMalloc(&t->unit_order[1], n * sizeof(uint32_t)));
     NB_ORDER_HIP(hipMalloc(&t->sort_vals, n * sizeof(uint32_t)));
     NB_ORDER_HIP(hipMalloc(&t->sort_keys, 2u * n));
     NB_ORDER_HIP(hipMalloc(&t->sort_offsets, sizeof(offsets)));
@@ -90,28 +96,39 @@ nb_status unit_order_alloc(nb_target *t) {
     t->cost_frames = 0;
     t->view_frames = 0;
     t->order_valid = false;
+    t->order_cur = 0;
+    t->order_pending = false;
+    t->rebuild_wanted = false;
     t->cost_scene = nullptr;
     return NB_OK;
 }
 
 void unit_order_free(nb_target *t) {
     hipFree(t->unit_cost);
-    hipFree(t->unit_order);
+    hipFree(t->unit_order[0]);
+    hipFree(t->unit_order[1]);
     hipFree(t->sort_vals);
     hipFree(t->sort_keys);
     hipFree(t->sort_offsets);
     hipFree(t->sort_tmp);
-    t->unit_cost = t->unit_order = t->sort_vals = nullptr;
+    t->unit_cost = t->sort_vals = nullptr;
+    t->unit_order[0] = t->unit_order[1] = nullptr;
     t->sort_keys = nullptr;
     t->sort_offsets = nullptr;
     t->sort_tmp = nullptr;
 }
 
 nb_status unit_order_prepare(nb_target *t, const nb_scene *scene, const nb_camera *cam, const nb_globals *g,
-                             RenderParams &p, hipStream_t s) {
+                             RenderParams &p, hipStream_t s, uint64_t visible_sorts) {
     p.unit_order = nullptr;
     p.unit_cost = nullptr;
     if (!order_enabled() || !t->unit_cost || t->n_units == 0) return NB_OK;
+    if (t->order_pending && t->pending_sort <= visible_sorts) {
+        // the coming launch follows the sort that wrote the other buffer: use it from now on
+        t->order_cur ^= 1u;
+        t->order_pending = false;
+        t->order_valid = true;
+    }
     const bool same_paths = scene == t->cost_scene && g->bounces == t->cost_bounces &&
                             g->contribution_factor == t->cost_contribution;
     if (!same_paths || std::memcmp(cam, &t->cost_camera, sizeof(nb_camera)) != 0) {
@@ -122,17 +139,21 @@ nb_status unit_order_prepare(nb_target *t, const nb_scene *scene, const nb_camer
         hipLaunchKernelGGL(shift_kernel, dim3((t->n_units + 255u) / 256u), dim3(256), 0, s, t->unit_cost, t->n_units,
                            shift);
         NB_ORDER_HIP(hipGetLastError());
-        t->order_dirty = true;
         t->cost_frames = same_paths ? t->cost_frames / 2u : 0u;
         t->view_frames = 0;
-        if (!same_paths) t->order_valid = false;
+        if (!same_paths) {
+            // an order (or one being built) from other paths would mislead
+            t->order_valid = false;
+            t->order_pending = false;
+            t->rebuild_wanted = false;
+        }
         t->cost_scene = scene;
         t->cost_camera = *cam;
         t->cost_bounces = g->bounces;
         t->cost_contribution = g->contribution_factor;
     }
     p.unit_cost = t->unit_cost;
-    p.unit_order = t->order_valid ? t->unit_order : nullptr;
+    p.unit_order = t->order_valid ? t->unit_order[t->order_cur] : nullptr;
     return NB_OK;
 }
 
@@ -148,18 +169,29 @@ nb_status unit_order_update(nb_target *t, const RenderParams &p, uint32_t n_fram
     const uint64_t first_pow2 = t->order_valid ? 2u : 1u;
     uint64_t pow2 = first_pow2;
     while (pow2 <= lo) pow2 <<= 1;                      // the first power of two after lo
-    const bool rebuild = pow2 <= hi || (lo / 64u + 1u) * 64u <= hi;
+    const bool rebuild = pow2 <= hi || (lo / 64u + 1u) * 64u <= hi || t->rebuild_wanted;
     if (!rebuild || t->cost_frames == 0) return NB_OK;
+    if (t->order_pending) {
+        // the other buffer still waits to be switched to: rebuild after that
+        t->rebuild_wanted = true;
+        return NB_OK;
+    }
+    // Into the buffer no launch in flight reads: launches that began before the switch to the
+    // current buffer have all finished (each is followed on `s` by its mix or is itself on `s`,
+    // and this call's own launch is already enqueued), and launches from now on read the
+    // current buffer until they follow this sort.
     const uint32_t n = t->n_units;
+    uint32_t *out = t->unit_order[t->order_cur ^ 1u];
     hipLaunchKernelGGL(order_keys_kernel, dim3((n + 255u) / 256u), dim3(256), 0, s, t->unit_cost, n, t->cost_frames,
                        order_shift(), t->sort_keys, t->sort_vals);
     NB_ORDER_HIP(hipGetLastError());
     size_t tmp = t->sort_tmp_bytes;
     NB_ORDER_HIP(hipcub::DeviceSegmentedRadixSort::SortPairsDescending(
-        t->sort_tmp, tmp, t->sort_keys, t->sort_keys + n, t->sort_vals, t->unit_order, static_cast<int>(n),
+        t->sort_tmp, tmp, t->sort_keys, t->sort_keys + n, t->sort_vals, out, static_cast<int>(n),
         static_cast<int>(kBands), t->sort_offsets, t->sort_offsets + 1, 0, 8, s));
-    t->order_dirty = true;
-    t->order_valid = true;
+    t->order_pending = true;
+    t->pending_sort = ++t->ctx->order_sorts;
+    t->rebuild_wanted = false;
     return NB_OK;
 }
 

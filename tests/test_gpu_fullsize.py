@@ -169,3 +169,25 @@ def test_fullsize_pipelined_equals_sequential(full):
     assert np.isfinite(images[0]).all()
     np.testing.assert_array_equal(images[0].view(np.uint32), images[1].view(np.uint32))
     assert counts[0] == counts[1]
+
+
+def test_fullsize_pipelined_moving_camera_equals_sequential(full):
+    """A camera that moves every frame: cost halving and order rebuilds overlap the previous
+    frame's paths (atomic costs, two order buffers)."""
+    nb, ctx, sb, env, cam, osc, faces, P = full
+    print("case: 1920x1080, 6 frames with a moving camera, pipelined vs sequential", flush=True)
+    images = []
+    for pipe in ("1", "0"):
+        os.environ["NB_PIPELINE"] = pipe
+        try:
+            pt = _tracer(nb, ctx, sb, env, cam, P)
+            for k in range(6):
+                eye = (P["eye"][0] + 0.5 * k, P["eye"][1], P["eye"][2] + 0.3 * k)
+                c = nb.CameraController(world_position=eye, target=P["target"], aspect_ratio=WIDTH / HEIGHT)
+                pt.camera, pt._camera_data = c, c.data()   # no invalidation: every frame counts
+                pt.dispatch()
+            images.append(pt.output_texture())
+        finally:
+            del os.environ["NB_PIPELINE"]
+    assert np.isfinite(images[0]).all()
+    np.testing.assert_array_equal(images[0].view(np.uint32), images[1].view(np.uint32))

@@ -142,3 +142,40 @@ def test_pipelined_render_timing(nb, ctx, small_gpu):
         finally:
             ctx.enable_render_timing(False)
     assert 0.0 < ms < 1000.0
+
+
+def _moving(nb, ctx, sb, env, pipeline, frames=12, size=(128, 96)):
+    """A camera that moves every second frame: the costs are halved before those frames and
+    orders are rebuilt and switched to while the previous frame's paths are in flight.  The
+    camera data changes without invalidating (the frames keep accumulating), so the final
+    image depends on every frame; one move in the middle invalidates as CameraController
+    updates do."""
+    from nbounce_amd import scenegen
+    P = scenegen.SMALL
+    out = []
+    with _Env(NB_PIPELINE="1" if pipeline else "0"):
+        pt, _ = _pathtracer(nb, ctx, sb, env, size, P["eye"], P["target"])
+        ctx.synchronize()
+        for k in range(frames):
+            eye = (P["eye"][0] + 0.05 * k, P["eye"][1], P["eye"][2] - 0.03 * k)
+            cam = nb.CameraController(world_position=eye, target=P["target"], aspect_ratio=size[0] / size[1])
+            if k == frames // 2:
+                pt.update(cam)
+            else:
+                pt.camera, pt._camera_data = cam, cam.data()
+            pt.dispatch()
+            pt.dispatch()
+            if k % 4 == 3:
+                out.append(pt.output_texture())
+        out.append(pt.output_texture())
+    return out
+
+
+def test_pipelined_moving_camera_equals_sequential(nb, ctx, small_gpu):
+    w, sb, env = small_gpu
+    print("case: small scene 128x96, camera moving every second frame, pipelined vs sequential", flush=True)
+    seq = _moving(nb, ctx, sb, env, pipeline=False)
+    pipe = _moving(nb, ctx, sb, env, pipeline=True)
+    for s, p in zip(seq, pipe):
+        assert np.isfinite(s).all()
+        np.testing.assert_array_equal(s.view(np.uint32), p.view(np.uint32))
