@@ -426,6 +426,7 @@ def cache_equal_error(train_paths=1 << 16, lr=1e-3, max_batch=None, frames=None,
     scene = nb.Scene(v, i, pr)
     blas, tlas, inst = nb.build_scene(scene)            # the reference's per-primitive BLAS, as the oracle
     ctx = nb.Context(0)
+    ctx.enable_counters(False)     # both tracers timed as the renderer is: counter-free kernels
     sb = nb.SceneBuffers(ctx, blas, tlas, inst, scene.vertices, scene.indices)
     env = nb.EnvMap(ctx, scenegen.sky_cube(P["env_face"]))
     W, H = P["width"], P["height"]

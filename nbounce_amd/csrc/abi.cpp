@@ -332,6 +332,8 @@ nb_status check_render_args(nb_context *ctx, const nb_scene *scene, nb_target *t
     if (scene->ctx != ctx || target->ctx != ctx || env->ctx != ctx)
         return set_error(NB_ERR_INVALID_ARGUMENT, "scene, target and environment must belong to the context");
     if (!ctx->sobol) return set_error(NB_ERR_INVALID_ARGUMENT, "no Sobol table set (nb_sobol_set)");
+    if (g->bounces > kMaxBounceLimit)
+        return set_error(NB_ERR_INVALID_ARGUMENT, "bounces %u exceeds %u", g->bounces, kMaxBounceLimit);
     return NB_OK;
 }
 
@@ -767,20 +769,23 @@ nb_status nb_envmap_create_latlong(nb_context *ctx, const float *texels, uint32_
     *out = nullptr;
     NB_TRY(use_device(ctx));
     // one border texel on every side: the columns wrap across the u seam, the rows repeat the
-    // pole rows, so the lookup's four taps never leave the padded map
+    // pole rows, so the lookup's four taps never leave the padded map; a header texel ahead of
+    // them holds the size
     const uint32_t pw = width + 2, ph = height + 2;
     std::vector<float4> padded;
     try {
-        padded.resize(static_cast<size_t>(pw) * ph);
+        padded.resize(static_cast<size_t>(pw) * ph + 1);
     } catch (const std::bad_alloc &) {
         return set_error(NB_ERR_OUT_OF_MEMORY, "host allocation failed");
     }
+    const uint32_t header[4] = {width, height, 0u, 0u};
+    std::memcpy(&padded[0], header, sizeof(header));
     for (uint32_t j = 0; j < ph; ++j) {
         const uint32_t sj = j == 0 ? 0u : (j == ph - 1 ? height - 1 : j - 1);
         for (uint32_t i = 0; i < pw; ++i) {
             const uint32_t si = i == 0 ? width - 1 : (i == pw - 1 ? 0u : i - 1);
             const float *t = texels + (static_cast<size_t>(sj) * width + si) * 4;
-            padded[static_cast<size_t>(j) * pw + i] = make_float4(t[0], t[1], t[2], t[3]);
+            padded[1 + static_cast<size_t>(j) * pw + i] = make_float4(t[0], t[1], t[2], t[3]);
         }
     }
     nb_envmap *m = new nb_envmap();
@@ -803,7 +808,8 @@ nb_status nb_envmap_create_latlong(nb_context *ctx, const float *texels, uint32_
 
 nb_status nb_render_options_set(nb_context *ctx, const nb_render_options *options) {
     if (!ctx || !options) return set_error(NB_ERR_INVALID_ARGUMENT, "null argument");
-    ctx->rr_first = options->roulette_first_bounce;
+    // (beyond the largest bounce limit a render accepts, it means no roulette at all)
+    ctx->rr_first = std::min(options->roulette_first_bounce, kMaxBounceLimit);
     return NB_OK;
 }
 

@@ -465,6 +465,8 @@ nb_status nb_cache_training_records(nb_context *ctx, const nb_scene *scene, uint
     if (g->sample == 0) return set_error(NB_ERR_INVALID_ARGUMENT, "samples are 1-based");
     if (!width || !height || width % 8 || height % 8 || train_paths == 0)
         return set_error(NB_ERR_INVALID_ARGUMENT, "need a nonzero multiple-of-8 size and training paths");
+    if (g->bounces > kMaxBounceLimit)
+        return set_error(NB_ERR_INVALID_ARGUMENT, "bounces %u exceeds %u", g->bounces, kMaxBounceLimit);
     C_HIP(hipSetDevice(ctx->device));
     const size_t npix = static_cast<size_t>(width) * height;
     CacheFrame f{};
@@ -508,6 +510,8 @@ nb_status nb_render_cached(nb_context *ctx, const nb_scene *scene, nb_target *ta
         return set_error(NB_ERR_INVALID_ARGUMENT, "all objects must belong to the context");
     if (!ctx->sobol) return set_error(NB_ERR_INVALID_ARGUMENT, "no Sobol table set (nb_sobol_set)");
     if (g->sample == 0) return set_error(NB_ERR_INVALID_ARGUMENT, "samples are 1-based");
+    if (g->bounces > kMaxBounceLimit)
+        return set_error(NB_ERR_INVALID_ARGUMENT, "bounces %u exceeds %u", g->bounces, kMaxBounceLimit);
     C_HIP(hipSetDevice(ctx->device));
     if (ctx->render_timing) C_HIP(hipEventRecord(ctx->ev_start, ctx->stream));
     C_TRY(cache_render_frame(ctx, scene->dev, dev_env(env), target->pixels, target->width,

@@ -43,6 +43,9 @@ constexpr uint32_t kLeafMaxInline = 31u;
 constexpr uint32_t kNoRecord = 0xFFFFFFFFu;
 constexpr uint32_t kMaxTreeDepth = 32;  // reference MAX_DEPTH (bvh.rs:153)
 constexpr uint32_t kStackEntries = 2 * kMaxTreeDepth;
+// The largest bounce limit a render accepts: the kernels keep it in 16 bits beside the first
+// roulette bounce (PathConst::bounces).
+constexpr uint32_t kMaxBounceLimit = 0xFFFFu;
 
 // Device counters: kCounterSlots slots (one per XCD), kCounterStride u64 apart (4 KiB, so the
 // slots' atomics do not share a cache line or memory channel), each of kCounterWords u64:
@@ -77,11 +80,12 @@ struct DevScene {
 
 struct DevEnv {
     // cube (n > 0): 6 faces of (n+2)^2 texels, one texel of seamless border around each face;
-    // lat-long (n == 0): height+2 rows of width+2 texels, the border columns wrapped from the
-    // opposite side and the border rows repeating the edge rows
+    // lat-long (n == 0): a header texel holding (width, height) as integers, then height+2 rows
+    // of width+2 texels, the border columns wrapped from the opposite side and the border rows
+    // repeating the edge rows.  (The size sits in memory, read only on a lookup, so that the
+    // kernels do not hold it in registers.)
     const float4 *texels;
     uint32_t n;
-    uint32_t width, height;   // lat-long size
 };
 
 struct RenderParams {
@@ -287,12 +291,12 @@ struct nb_envmap {
 };
 
 inline nb::DevEnv dev_env(const nb_envmap *e) {
-    return e ? nb::DevEnv{e->texels, e->n, e->width, e->height} : nb::DevEnv{nullptr, 0u, 0u, 0u};
+    return e ? nb::DevEnv{e->texels, e->n} : nb::DevEnv{nullptr, 0u};
 }
 
 // Device bytes of an environment map's padded texels.
 inline uint64_t envmap_device_bytes(const nb_envmap *e) {
-    return e->n ? 6ull * (e->n + 2) * (e->n + 2) * 16 : static_cast<uint64_t>(e->width + 2) * (e->height + 2) * 16;
+    return e->n ? 6ull * (e->n + 2) * (e->n + 2) * 16 : (static_cast<uint64_t>(e->width + 2) * (e->height + 2) + 1) * 16;
 }
 
 

@@ -395,7 +395,8 @@ __device__ F3 latlong_sample(const DevEnv &E, F3 d) {
     const float r = sqrtf(d.x * d.x + d.z * d.z);
     const float u = atan2_f(d.z, d.x) * 0.15915494309189535f + 0.5f;
     const float v = atan2_f(r, d.y) * 0.31830988618379067f;
-    const float w = static_cast<float>(E.width), h = static_cast<float>(E.height);
+    const uint4 dims = *reinterpret_cast<const uint4 *>(E.texels);   // header texel: width, height
+    const float w = static_cast<float>(dims.x), h = static_cast<float>(dims.y);
     const float x = u * w - 0.5f, y = v * h - 0.5f;
     float fx = floorf(x), fy = floorf(y);
     if (!(fx >= -1.0f)) fx = -1.0f;
@@ -403,9 +404,9 @@ __device__ F3 latlong_sample(const DevEnv &E, F3 d) {
     if (!(fy >= -1.0f)) fy = -1.0f;
     if (fy > h - 1.0f) fy = h - 1.0f;
     const float a = x - fx, b = y - fy;
-    const uint32_t pw = E.width + 2;
+    const uint32_t pw = dims.x + 2;
     const int i0 = static_cast<int>(fx) + 1, j0 = static_cast<int>(fy) + 1;
-    const float4 *base = E.texels + static_cast<size_t>(j0) * pw + i0;
+    const float4 *base = E.texels + 1 + static_cast<size_t>(j0) * pw + i0;
     const float4 t00 = base[0], t10 = base[1], t01 = base[pw], t11 = base[pw + 1];
     const float w00 = (1.0f - a) * (1.0f - b), w10 = a * (1.0f - b), w01 = (1.0f - a) * b, w11 = a * b;
     return f3(((w00 * t00.x + w10 * t10.x) + w01 * t01.x) + w11 * t11.x,
@@ -518,9 +519,11 @@ struct PathConst {
     DevScene scene;
     DevEnv env;
     SobolView sobol;
-    uint32_t sample, bounces;
+    uint32_t sample;
+    // the bounce limit in bits 0..15 and, above it, the first bounce with Russian roulette (the
+    // reference: 0); packed so that the capped kernels hold one register less
+    uint32_t bounces;
     float contribution_factor;
-    uint32_t rr_first;       // first bounce with Russian roulette (the reference: 0)
     float4 shift;
 };
 
@@ -625,10 +628,10 @@ __device__ StepResult shade(const PathConst &pc, uint32_t bounce, const Hit &h, 
         F3 diff = f3((om * albedo.x) * resp, (om * albedo.y) * resp, (om * albedo.z) * resp);
         thr = mul(thr, divs(diff, p_diff));
     }
-    // Russian roulette, pathtracing.wgsl:227-236, from bounce rr_first on (the reference: 0;
+    // Russian roulette, pathtracing.wgsl:227-236, from bounce bounces >> 16 on (the reference: 0;
     // before it every path continues with its throughput as it is)
-    if (bounce >= pc.rr_first) {
-        float p_cont = fminf(1.0f - pow8(static_cast<float>(bounce) / static_cast<float>(pc.bounces)), 1.0f);
+    if (bounce >= (pc.bounces >> 16)) {
+        float p_cont = fminf(1.0f - pow8(static_cast<float>(bounce) / static_cast<float>(pc.bounces & 0xFFFFu)), 1.0f);
         p_cont *= fminf(luminance(thr) * pc.contribution_factor, 1.0f);
         if (sob0.z < p_cont) {
             thr = divs(thr, p_cont);
@@ -660,7 +663,7 @@ __device__ __forceinline__ StepResult bounce_step(const PathConst &pc, uint32_t 
 template <int K>
 __device__ F3 sample_path(const PathConst &pc, Ray ray, Stack<K> &st, LaneCounters &c) {
     F3 thr = f3(1.0f, 1.0f, 1.0f), out = f3(0.0f, 0.0f, 0.0f);
-    for (uint32_t bounce = 0; bounce <= pc.bounces; ++bounce)
+    for (uint32_t bounce = 0; bounce <= (pc.bounces & 0xFFFFu); ++bounce)
         if (bounce_step<K>(pc, bounce, ray, thr, out, st, c) == kDone) return out;
     return f3(0.0f, 0.0f, 0.0f);
 }

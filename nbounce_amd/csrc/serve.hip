@@ -124,7 +124,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5))) void se
                 Vertex dummy{};
                 pc.shift = hash4f(x, y, x, y);
                 r = shade<false>(pc, bounce, h, ray, thr, out, dummy);
-                if (r == kContinue && bounce == pc.bounces) { r = kDone; out = f3(0.0f, 0.0f, 0.0f); }
+                if (r == kContinue && bounce == (pc.bounces & 0xFFFFu)) { r = kDone; out = f3(0.0f, 0.0f, 0.0f); }
             }
             if (r == kContinue) {
                 ++bounce;
@@ -177,9 +177,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5))) void se
                     y = (tile / tiles_x) * 8u + (kk >> 3);
                     path = P->path_base + k * pixels + y * width + x;
                     pc.sample = P->first_sample + k;
-                    pc.bounces = R->bounces;
+                    pc.bounces = R->bounces | (R->rr_first << 16);
                     pc.contribution_factor = R->contribution_factor;
-                    pc.rr_first = R->rr_first;
                     const float4 jit = sobol_fetch(pc.sobol, pc.sample * S.sobol_stride, hash4f(x, y, x, y));
                     ray = camera_ray(R->c2w, width, R->height, x, y, jit.x, jit.y);
                     thr = f3(1.0f, 1.0f, 1.0f);

@@ -47,9 +47,8 @@ __global__ __launch_bounds__(64) void render_kernel(RenderParams p) {
     pc.scene = p.scene;
     pc.env = p.env;
     pc.sobol = SobolView{p.sobol, p.sobol_entries, p.sobol_stride};
-    pc.bounces = p.bounces;
+    pc.bounces = p.bounces | (p.rr_first << 16);
     pc.contribution_factor = p.contribution_factor;
-    pc.rr_first = p.rr_first;
     pc.shift = hash4f(x, y, x, y);
     F3 col = f3(0.0f, 0.0f, 0.0f), s = f3(0.0f, 0.0f, 0.0f);
     for (uint32_t f = 0; f < p.n_frames; ++f) {
@@ -125,9 +124,8 @@ __global__ __launch_bounds__(64) void render_pt_kernel(RenderParams p) {
     pc.scene = p.scene;
     pc.env = p.env;
     pc.sobol = SobolView{p.sobol, p.sobol_entries, p.sobol_stride};
-    pc.bounces = p.bounces;
+    pc.bounces = p.bounces | (p.rr_first << 16);
     pc.contribution_factor = p.contribution_factor;
-    pc.rr_first = p.rr_first;
 
     // wave-uniform pixel stream: pixels [pos, end) of the wave's current units come next; the
     // first units are this wave's static share of its XCD's band
