@@ -6,6 +6,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstdarg>
 #include <cstdlib>
@@ -141,6 +142,104 @@ void make_pair(const nb_bvh_node &l, uint32_t lref, const nb_bvh_node *r, uint32
     std::memcpy(&q[3], w, 16);
 }
 
+// ---- TLAS refit ------------------------------------------------------------------------------
+// The reference bounds an instance by the 8 transformed corners of its BLAS root box
+// (scene.rs:239-260).  For a rotated mesh that box is loose (a rotated cube's corners reach up
+// to sqrt(3) times further along an axis than its contents), so rays enter instances and TLAS
+// nodes they miss.  The kernels traverse the same TLAS topology with boxes refitted bottom-up to
+// each instance's transformed vertices, computed in double precision and padded outward by
+// 2^-14 of the coordinates' magnitude (well above the float error of the ray transform and the
+// slab test, so every ray that hits an instance's geometry still enters it at a distance no
+// greater than its hit).  The closest hit is the same; only which nodes are visited on the way
+// changes.  A bound that is not finite keeps the reference box.  NB_TLAS_REFIT=0 keeps the
+// reference boxes, read at scene creation.
+bool tlas_refit_enabled() {
+    const char *e = std::getenv("NB_TLAS_REFIT");
+    return !(e && std::strcmp(e, "0") == 0);
+}
+
+nb_status tlas_refit(const nb_bvh_node *blas, const nb_bvh_node *tlas, uint32_t n_tlas, const nb_instance *inst,
+                     uint32_t n_inst, const nb_vertex *vert, const uint32_t *idx, std::vector<nb_bvh_node> &out) {
+    out.assign(tlas, tlas + n_tlas);
+    if (!tlas_refit_enabled() || n_tlas == 0) return NB_OK;
+    try {
+        // the unique vertices each instanced BLAS root covers (its leaves' triangle range)
+        std::map<uint32_t, std::vector<uint32_t>> verts_of;
+        std::vector<uint32_t> stack;
+        for (uint32_t j = 0; j < n_inst; ++j) {
+            const uint32_t root = inst[j].node;
+            if (verts_of.count(root)) continue;
+            uint32_t lo = 0xFFFFFFFFu, hi = 0;
+            stack.assign(1, root);
+            while (!stack.empty()) {
+                const nb_bvh_node &n = blas[stack.back()];
+                stack.pop_back();
+                if (n.end > 0) {
+                    lo = std::min(lo, n.start);
+                    hi = std::max(hi, n.end);
+                } else {
+                    stack.push_back(n.start);
+                    stack.push_back(n.start + 1);
+                }
+            }
+            std::vector<uint32_t> &v = verts_of[root];
+            if (hi > lo) {
+                v.assign(idx + 3 * static_cast<size_t>(lo), idx + 3 * static_cast<size_t>(hi));
+                std::sort(v.begin(), v.end());
+                v.erase(std::unique(v.begin(), v.end()), v.end());
+            }
+        }
+        // each instance's padded world bounds (min xyz, max xyz); NaN when not finite
+        std::vector<std::array<float, 6>> ib(n_inst);
+        for (uint32_t j = 0; j < n_inst; ++j) {
+            const float *m = inst[j].local_to_world;   // column-major
+            double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+            for (uint32_t vi : verts_of[inst[j].node]) {
+                const float *p = vert[vi].position;
+                for (int a = 0; a < 3; ++a) {
+                    const double w = static_cast<double>(m[a]) * p[0] + static_cast<double>(m[4 + a]) * p[1] +
+                                     static_cast<double>(m[8 + a]) * p[2] + static_cast<double>(m[12 + a]);
+                    lo[a] = std::min(lo[a], w);
+                    hi[a] = std::max(hi[a], w);
+                }
+            }
+            bool ok = true;
+            for (int a = 0; a < 3; ++a) ok = ok && std::isfinite(lo[a]) && std::isfinite(hi[a]);
+            for (int a = 0; a < 3; ++a) {
+                if (!ok) { ib[j][a] = ib[j][3 + a] = NAN; continue; }
+                const double pad = std::ldexp(std::max({std::fabs(lo[a]), std::fabs(hi[a]), hi[a] - lo[a]}), -14);
+                ib[j][a] = std::nextafter(static_cast<float>(lo[a] - pad), -INFINITY);
+                ib[j][3 + a] = std::nextafter(static_cast<float>(hi[a] + pad), INFINITY);
+            }
+        }
+        // bottom-up: children follow their parents in the node array
+        for (uint32_t i = n_tlas; i-- > 0;) {
+            nb_bvh_node &n = out[i];
+            float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+            bool ok = true;
+            auto add = [&](const float *bmin, const float *bmax) {
+                for (int a = 0; a < 3; ++a) {
+                    ok = ok && std::isfinite(bmin[a]) && std::isfinite(bmax[a]);
+                    lo[a] = std::min(lo[a], bmin[a]);
+                    hi[a] = std::max(hi[a], bmax[a]);
+                }
+            };
+            if (n.end > 0) {
+                for (uint32_t j = n.start; j < n.end; ++j) add(ib[j].data(), ib[j].data() + 3);
+            } else {
+                add(out[n.start].min, out[n.start].max);
+                add(out[n.start + 1].min, out[n.start + 1].max);
+            }
+            if (!ok) continue;   // keeps the reference box
+            std::memcpy(n.min, lo, sizeof(lo));
+            std::memcpy(n.max, hi, sizeof(hi));
+        }
+    } catch (const std::bad_alloc &) {
+        return set_error(NB_ERR_OUT_OF_MEMORY, "host allocation failed");
+    }
+    return NB_OK;
+}
+
 nb_status relayout(const nb_bvh_node *blas, uint32_t n_blas, const nb_bvh_node *tlas, uint32_t n_tlas,
                    const nb_instance *inst, uint32_t n_inst, const nb_vertex *vert, uint32_t n_vert,
                    const uint32_t *idx, uint32_t n_idx, Relayout &R) {
@@ -155,6 +254,11 @@ nb_status relayout(const nb_bvh_node *blas, uint32_t n_blas, const nb_bvh_node *
     if (static_cast<uint64_t>(n_tris) * 48u > 0xFFFFFFFFull) return set_error(NB_ERR_UNSUPPORTED, "more than 89478485 triangles");
     for (uint32_t j = 0; j < n_inst; ++j)
         if (inst[j].node >= n_blas) return set_error(NB_ERR_BAD_LAYOUT, "instance %u: BLAS root %u out of range", j, inst[j].node);
+    // the TLAS the kernels traverse: the given topology with boxes refitted to the instances'
+    // transformed geometry (tlas_refit)
+    std::vector<nb_bvh_node> tl;
+    NB_TRY(tlas_refit(blas, tlas, n_tlas, inst, n_inst, vert, idx, tl));
+    tlas = tl.data();
 
     // Record ids in depth-first preorder of each tree (left child first), each tree preceded by
     // its root record: a record's left child is the next record, so descents stay in one cache
