@@ -191,3 +191,30 @@ def test_fullsize_pipelined_moving_camera_equals_sequential(full):
             del os.environ["NB_PIPELINE"]
     assert np.isfinite(images[0]).all()
     np.testing.assert_array_equal(images[0].view(np.uint32), images[1].view(np.uint32))
+
+
+def test_fullsize_tlas_refit_changes_no_pixel(full):
+    """The TLAS the kernels traverse has its boxes refitted to the instances' transformed
+    geometry (NB_TLAS_REFIT, read at scene creation): fewer instance entries and box tests,
+    the same closest hits, so the same image and ray count as the reference's boxes."""
+    nb, ctx, sb, env, cam, osc, faces, P = full
+    from nbounce_amd import scenegen
+    print("case: 1920x1080, 3 frames, refitted TLAS vs the reference's TLAS boxes", flush=True)
+    v, i, pr = scenegen.instanced_meshes(0)
+    scene = nb.Scene(v, i, pr)
+    blas, tlas, inst = nb.build_scene(scene, share_blas=True)
+    os.environ["NB_TLAS_REFIT"] = "0"
+    try:
+        sb_ref = nb.SceneBuffers(ctx, blas, tlas, inst, scene.vertices, scene.indices)
+    finally:
+        del os.environ["NB_TLAS_REFIT"]
+    counts, images = [], []
+    for buffers in (sb, sb_ref):
+        ctx.reset_counters()
+        images.append(_render(_tracer(nb, ctx, buffers, env, cam, P), 3))
+        counts.append(ctx.counters())
+    np.testing.assert_array_equal(images[0].view(np.uint32), images[1].view(np.uint32))
+    assert counts[0]["rays"] == counts[1]["rays"] and counts[0]["hits"] == counts[1]["hits"]
+    assert counts[0]["n_inst"] < 0.7 * counts[1]["n_inst"]
+    assert counts[0]["n_aabb"] < counts[1]["n_aabb"]
+    sb_ref.close()

@@ -111,3 +111,27 @@ def test_roulette_from_bounce_three_matches_oracle(nb, ctx, small):
     r3 = counts[3]["rays"] / counts[3]["samples"]
     print(f"rays per sample: roulette from bounce 0 {r0:.3f}, from bounce 3 {r3:.3f}")
     assert r3 > r0
+
+
+def test_bounce_limit_and_roulette_start_bounds(nb, ctx, small):
+    """Bounce limits above 65535 are refused (the kernels keep the limit in 16 bits beside the
+    first roulette bounce); a first roulette bounce beyond the limit means no roulette at all,
+    and a huge one is clamped to the same."""
+    from nbounce_amd import scenegen
+    w, sb = small
+    env = nb.EnvMap(ctx, scenegen.sky_cube(32))
+    print("case: small scene 64x64, bounces 70000 refused; roulette start 9 vs 2^20", flush=True)
+    pt, _ = _tracer(nb, ctx, sb, env)
+    pt.globals.bounces = 70000
+    with pytest.raises(nb.NbError):
+        pt.dispatch()
+    images = []
+    try:
+        for first in (9, 1 << 20):
+            ctx.set_render_options(roulette_first_bounce=first)
+            pt, _ = _tracer(nb, ctx, sb, env)
+            pt.dispatch()
+            images.append(pt.output_texture())
+    finally:
+        ctx.set_render_options(roulette_first_bounce=0)
+    np.testing.assert_array_equal(images[0].view(np.uint32), images[1].view(np.uint32))
