@@ -691,8 +691,8 @@ def main():
             # far from saturated: the limiter is the latency of the per-lane node fetches through
             # the texture-data path, with VALU issue beside it
             "bound": "hbm",
-            "limiter": "dependent node fetches through the texture-data path (TD busy 0.82, 40 % of it stalled "
-                       "on the L1) and VALU issue (0.59 busy); HBM counter traffic is hbm_counter_frac of peak",
+            "limiter": "dependent node fetches through the texture-data path (TD busy 0.79, half of it stalled "
+                       "on the L1) and VALU issue (about 0.6 busy); HBM counter traffic is hbm_counter_frac of peak",
             "hbm_counter_frac": hbm_frac,
             "achieved": round(achieved, 2),
             "peak": PEAK_HBM_GBS,

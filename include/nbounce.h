@@ -135,7 +135,10 @@ nb_status nb_build_scene(const nb_vertex *vertices, uint32_t n_vertices,
 /* ---- device scene: group-1 bindings blas, tlas, instances, vertices, indices ------------
  * (src/pathtracing/raytracing_sw.wgsl:33-37, uploaded at scene.rs:309-341).  The arrays are
  * validated (indices in range, children after parents, depth <= 32) and re-laid out for the
- * GPU; the caller's arrays are not kept. */
+ * GPU; the caller's arrays are not kept.  The TLAS keeps its topology, but the boxes the kernels
+ * traverse are refitted to each instance's transformed vertices (padded outward), so rays enter
+ * fewer instances they miss; the closest hits are the same.  NB_TLAS_REFIT=0 in the environment
+ * keeps the given boxes. */
 nb_status nb_scene_create(nb_context *ctx,
                           const nb_bvh_node *blas, uint32_t n_blas,
                           const nb_bvh_node *tlas, uint32_t n_tlas,
