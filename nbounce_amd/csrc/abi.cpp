@@ -163,12 +163,14 @@ nb_status tlas_refit(const nb_bvh_node *blas, const nb_bvh_node *tlas, uint32_t 
     out.assign(tlas, tlas + n_tlas);
     if (!tlas_refit_enabled() || n_tlas == 0) return NB_OK;
     try {
-        // the unique vertices each instanced BLAS root covers (its leaves' triangle range)
-        std::map<uint32_t, std::vector<uint32_t>> verts_of;
+        // the unique vertices each instanced BLAS root covers (its leaves' triangle range), kept
+        // per range: the reference's one-BLAS-per-instance layout has many roots over one range
+        std::map<uint32_t, uint64_t> range_of;                     // root -> lo << 32 | hi
+        std::map<uint64_t, std::vector<uint32_t>> verts_of;        // range -> vertex ids
         std::vector<uint32_t> stack;
         for (uint32_t j = 0; j < n_inst; ++j) {
             const uint32_t root = inst[j].node;
-            if (verts_of.count(root)) continue;
+            if (range_of.count(root)) continue;
             uint32_t lo = 0xFFFFFFFFu, hi = 0;
             stack.assign(1, root);
             while (!stack.empty()) {
@@ -182,7 +184,10 @@ nb_status tlas_refit(const nb_bvh_node *blas, const nb_bvh_node *tlas, uint32_t 
                     stack.push_back(n.start + 1);
                 }
             }
-            std::vector<uint32_t> &v = verts_of[root];
+            const uint64_t key = hi > lo ? (static_cast<uint64_t>(lo) << 32) | hi : 0u;
+            range_of[root] = key;
+            if (verts_of.count(key)) continue;
+            std::vector<uint32_t> &v = verts_of[key];
             if (hi > lo) {
                 v.assign(idx + 3 * static_cast<size_t>(lo), idx + 3 * static_cast<size_t>(hi));
                 std::sort(v.begin(), v.end());
@@ -194,7 +199,7 @@ nb_status tlas_refit(const nb_bvh_node *blas, const nb_bvh_node *tlas, uint32_t 
         for (uint32_t j = 0; j < n_inst; ++j) {
             const float *m = inst[j].local_to_world;   // column-major
             double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
-            for (uint32_t vi : verts_of[inst[j].node]) {
+            for (uint32_t vi : verts_of[range_of[inst[j].node]]) {
                 const float *p = vert[vi].position;
                 for (int a = 0; a < 3; ++a) {
                     const double w = static_cast<double>(m[a]) * p[0] + static_cast<double>(m[4 + a]) * p[1] +
