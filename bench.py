@@ -699,7 +699,8 @@ def main():
             "unit": "GB/s",
             "frac": round(achieved / PEAK_HBM_GBS, 5),
             "traffic": traffic,
-            "traffic_source": "rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE, scaled by a 256 MiB copy "
+            "traffic_source": "rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE (L2 to memory, Infinity Cache hits "
+                              "included: an upper bound on HBM bytes), scaled by a 256 MiB copy "
                               "(profiles/render_pt_kernel_instanced_1080p_traffic.md)" if traffic else None,
             "kernel": "render_pt_kernel" if os.environ.get("NB_RENDER", "pt") != "tile" else "render_kernel",
             "kernel_ms": round(kernel_ms, 4),
