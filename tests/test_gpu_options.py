@@ -135,3 +135,33 @@ def test_bounce_limit_and_roulette_start_bounds(nb, ctx, small):
     finally:
         ctx.set_render_options(roulette_first_bounce=0)
     np.testing.assert_array_equal(images[0].view(np.uint32), images[1].view(np.uint32))
+
+
+def test_small_configuration_as_written_matches_oracle(nb, ctx, small):
+    """The small configuration as specified, at its full size: 256x256, 16 spp, 8 bounces, Russian
+    roulette from bounce 3, a 512x256 lat-long sky.  All 65,536 pixels of the 16-spp image, rendered
+    in one nb_render_frames call (sample-parallel kernels) and by 16 pipelined nb_render calls,
+    against the oracle's 16 frames."""
+    from nbounce_amd import scenegen
+    w, sb = small
+    sky = scenegen.sky_latlong(512, 256)
+    env = nb.EnvMap.latlong(ctx, sky)
+    size = (256, 256)
+    print("case: small scene 256x256, 16 spp, roulette from bounce 3, 512x256 lat-long sky: one call "
+          "and 16 pipelined frames vs the oracle", flush=True)
+    try:
+        ctx.set_render_options(roulette_first_bounce=3)
+        oracle.set_roulette_first(3)
+        a, cam = _tracer(nb, ctx, sb, env, size)
+        assert a.dispatch_frames(16) == 16
+        b, _ = _tracer(nb, ctx, sb, env, size)
+        for _ in range(16):
+            b.dispatch()
+        got_a, got_b = a.output_texture(), b.output_texture()
+        ref = _oracle_frames(w, cam, oracle.Env.latlong(sky), 16, size=size)
+    finally:
+        ctx.set_render_options(roulette_first_bounce=0)
+        oracle.set_roulette_first(0)
+    np.testing.assert_array_equal(got_a.view(np.uint32), got_b.view(np.uint32))
+    assert (got_a[..., :3].sum(-1) > 0).mean() > 0.5
+    _compare(got_a, ref)
