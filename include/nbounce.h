@@ -11,8 +11,10 @@
  *     thread-local message.  No call aborts the process.
  *   - Host arrays stay owned by the caller; the library copies what it needs during the call.
  *   - A context is externally synchronized (one host thread at a time).  Rendering calls are
- *     asynchronous on the context's HIP stream; nb_target_read() and nb_context_synchronize()
- *     wait for them.
+ *     asynchronous on the context's HIP stream (and, for pipelined nb_render frames, its two
+ *     side streams); nb_target_read() and nb_context_synchronize() wait for them.  A destroy
+ *     function waits for the work that may still read what it frees: for scenes, environment
+ *     maps, targets and nb_sobol_set()'s old table, all of the context's streams.
  *   - Destroy functions accept NULL.  Scenes, targets and environment maps keep their context
  *     alive: nb_context_destroy() takes effect when the last of them is destroyed.
  */

@@ -650,6 +650,20 @@ nb_status timed_frames(nb_context *ctx, const RenderParams &p) {
 
 static void release_context(nb_context *ctx);
 
+// Waits for all of the context's work: its stream and both side streams of the pipelined frames.
+// Each side-stream path kernel is normally followed by its mix on the context stream, which
+// waits for it; the side streams are synchronised as well so that nothing a path kernel reads is
+// freed under it even after an error left a frame without its mix.
+static hipError_t ctx_quiesce(nb_context *ctx) {
+    hipError_t e = hipStreamSynchronize(ctx->stream);
+    for (int i = 0; i < 2; ++i)
+        if (ctx->pipe_stream[i]) {
+            const hipError_t f = hipStreamSynchronize(ctx->pipe_stream[i]);
+            if (e == hipSuccess) e = f;
+        }
+    return e;
+}
+
 extern "C" {
 
 const char *nb_last_error(void) { return g_error.c_str(); }
@@ -731,7 +745,7 @@ nb_status nb_context_destroy(nb_context *ctx) {
 nb_status nb_context_synchronize(nb_context *ctx) {
     if (!ctx) return set_error(NB_ERR_INVALID_ARGUMENT, "null context");
     NB_TRY(use_device(ctx));
-    NB_HIP(hipStreamSynchronize(ctx->stream));
+    NB_HIP(ctx_quiesce(ctx));
     return NB_OK;
 }
 
@@ -813,7 +827,7 @@ nb_status nb_scene_create(nb_context *ctx, const nb_bvh_node *blas, uint32_t n_b
 nb_status nb_scene_destroy(nb_scene *scene) {
     if (!scene) return NB_OK;
     hipSetDevice(scene->ctx->device);
-    hipStreamSynchronize(scene->ctx->stream);
+    ctx_quiesce(scene->ctx);
     hipFree(scene->block);
     nb_context *ctx = scene->ctx;
     delete scene;
@@ -838,7 +852,7 @@ nb_status nb_sobol_set(nb_context *ctx, const float *table, uint32_t rows, uint3
     uint64_t entries = static_cast<uint64_t>(rows) * dims;
     if (entries > 0x7FFFFFFFull) return set_error(NB_ERR_INVALID_ARGUMENT, "table too large");
     NB_TRY(use_device(ctx));
-    NB_HIP(hipStreamSynchronize(ctx->stream));
+    NB_HIP(ctx_quiesce(ctx));
     float4 *d = nullptr;
     NB_HIP(hipMalloc(&d, entries * 16));
     hipError_t e = hipMemcpy(d, table, entries * 16, hipMemcpyHostToDevice);
@@ -925,7 +939,7 @@ nb_status nb_render_options_set(nb_context *ctx, const nb_render_options *option
 nb_status nb_envmap_destroy(nb_envmap *env) {
     if (!env) return NB_OK;
     hipSetDevice(env->ctx->device);
-    hipStreamSynchronize(env->ctx->stream);
+    ctx_quiesce(env->ctx);
     hipFree(env->texels);
     nb_context *ctx = env->ctx;
     delete env;
@@ -973,13 +987,8 @@ nb_status nb_target_create(nb_context *ctx, uint32_t width, uint32_t height, nb_
 nb_status nb_target_destroy(nb_target *t) {
     if (!t) return NB_OK;
     hipSetDevice(t->ctx->device);
-    hipStreamSynchronize(t->ctx->stream);
-    for (int i = 0; i < 2; ++i) {
-        // (each path kernel's mix follows it on the context stream; this also covers a path
-        // kernel whose mix was never enqueued)
-        if (t->ctx->pipe_stream[i]) hipStreamSynchronize(t->ctx->pipe_stream[i]);
-        hipFree(t->samples[i]);
-    }
+    ctx_quiesce(t->ctx);
+    for (int i = 0; i < 2; ++i) hipFree(t->samples[i]);
     hipFree(t->pixels);
     unit_order_free(t);
     nb_context *ctx = t->ctx;
@@ -1084,7 +1093,7 @@ nb_status nb_stream_timer_stop(nb_context *ctx, float *ms) {
 nb_status nb_counters_get(nb_context *ctx, nb_counters *out) {
     if (!ctx || !out) return set_error(NB_ERR_INVALID_ARGUMENT, "null argument");
     NB_TRY(use_device(ctx));
-    NB_HIP(hipStreamSynchronize(ctx->stream));
+    NB_HIP(ctx_quiesce(ctx));
     std::vector<unsigned long long> v(kCounterBytes / sizeof(unsigned long long));
     unsigned long long sum[kCounterWords] = {};
     NB_HIP(hipMemcpy(v.data(), ctx->counters, kCounterBytes, hipMemcpyDeviceToHost));
