@@ -4,7 +4,8 @@ frames' paths overlap.  These tests pin it against the same calls rendered one l
 another on the context stream (NB_PIPELINE=0, read at every call) — bit for bit and count for
 count — across every case where the ordering rules matter: the work-unit order rebuilt between
 frames, a camera change (costs halved), a counter reset, two targets interleaved, a one-launch
-nb_render_frames and a target write between pipelined frames, custom and zero weights."""
+nb_render_frames and a target write between pipelined frames, custom and zero weights; and
+destroy calls right after pipelined frames."""
 import os
 
 import numpy as np
@@ -179,3 +180,27 @@ def test_pipelined_moving_camera_equals_sequential(nb, ctx, small_gpu):
     for s, p in zip(seq, pipe):
         assert np.isfinite(s).all()
         np.testing.assert_array_equal(s.view(np.uint32), p.view(np.uint32))
+
+
+def test_destroy_right_after_pipelined_frames(nb, ctx, small_gpu):
+    """A scene and an environment map destroyed straight after pipelined nb_render calls, with no
+    synchronisation in between: the destroy calls wait for the frames still in flight on the
+    side streams, and the image is the one rendered with both kept alive."""
+    w, sb, env = small_gpu
+    from nbounce_amd import scenegen
+    P = scenegen.SMALL
+    print("case: small scene 128x96, scene and environment destroyed right after 3 pipelined frames", flush=True)
+    with _Env(NB_PIPELINE="1"):
+        ref, _ = _pathtracer(nb, ctx, sb, env, (128, 96), P["eye"], P["target"])
+        for _ in range(3):
+            ref.dispatch()
+        expected = ref.output_texture()
+        sb2 = nb.SceneBuffers(ctx, w["blas"], w["tlas"], w["instances"], w["vertices"], w["indices"])
+        env2 = nb.EnvMap(ctx, scenegen.sky_cube(128))
+        pt, _ = _pathtracer(nb, ctx, sb2, env2, (128, 96), P["eye"], P["target"])
+        for _ in range(3):
+            pt.dispatch()
+        env2.close()
+        sb2.close()
+        got = pt.output_texture()
+    np.testing.assert_array_equal(expected.view(np.uint32), got.view(np.uint32))
