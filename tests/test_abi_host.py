@@ -12,6 +12,7 @@ import pytest
 import oracle
 from nbounce_amd import _lib, layout, scenegen
 import nbounce_amd as nb
+from tests import _workloads as W
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
@@ -66,6 +67,19 @@ def test_builder_is_bit_exact_with_the_oracle_small(seed):
 
 def test_builder_is_bit_exact_with_the_oracle_rotated_instances():
     v, i, pr = scenegen.instanced_meshes(3, n_instances=96, extent=40.0)
+    sc = nb.Scene(v, i.copy(), pr)
+    blas, tlas, inst = nb.build_scene(sc)
+    ob, ot, oi, oidx = oracle.build_scene(v, i, pr)
+    assert blas.tobytes() == ob.tobytes() and tlas.tobytes() == ot.tobytes()
+    assert inst.tobytes() == oi.tobytes() and np.array_equal(sc.indices, oidx)
+
+
+@pytest.mark.parametrize("case", range(W.RANDOM_CASES))
+def test_builder_is_bit_exact_with_the_oracle_random(case):
+    """The seeded random scenes of the randomised GPU parity cases (spheres in boxes of random
+    size, random numbers of rotated and scaled instances): the library's builder and the
+    oracle's produce the same bytes."""
+    v, i, pr = W.random_case(case)["vit"]
     sc = nb.Scene(v, i.copy(), pr)
     blas, tlas, inst = nb.build_scene(sc)
     ob, ot, oi, oidx = oracle.build_scene(v, i, pr)

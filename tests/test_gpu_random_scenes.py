@@ -17,7 +17,7 @@ from tests import _workloads as W
 pytestmark = pytest.mark.gpu
 
 REL_TOL = 1e-3
-CASES = 24
+CASES = W.RANDOM_CASES
 
 
 @pytest.fixture(scope="module")
@@ -33,47 +33,9 @@ def ctx(nb):
     c.close()
 
 
-def _case(case):
-    """The scene, camera and settings of one case (deterministic in `case`)."""
-    rng = np.random.default_rng(7000 + case)
-    if case % 2 == 0:
-        box = float(rng.uniform(2.0, 30.0))
-        n = int(rng.integers(1, 41))
-        v, i, pr = scenegen.spheres_in_box(seed=case, n_spheres=n, n_emissive=int(rng.integers(0, n + 1)),
-                                           subdivisions=int(rng.integers(1, 4)), box=box)
-        lo, hi = np.zeros(3), np.full(3, box)
-        what = f"{n} spheres in a {box:.1f} box"
-    else:
-        extent = float(rng.uniform(5.0, 300.0))
-        n = int(rng.integers(1, 49))
-        v, i, pr = scenegen.instanced_meshes(seed=case, n_instances=n, extent=extent,
-                                             emissive_every=int(rng.integers(1, 9)))
-        lo, hi = np.zeros(3), np.full(3, extent)
-        what = f"{n} instanced meshes over {extent:.0f}"
-    span = hi - lo
-    eye = lo + span * rng.uniform(-0.4, 1.4, 3) if rng.random() < 0.5 else lo + span * rng.uniform(0.1, 0.9, 3)
-    if rng.random() < 0.3:
-        # exactly along the x or z axis (the up vector is +y)
-        axis = np.zeros(3)
-        axis[int(rng.choice([0, 2]))] = float(rng.choice([-1.0, 1.0]))
-        target = eye + axis * float(span.max())
-        what += ", axis-aligned view"
-    else:
-        target = lo + span * rng.uniform(0.2, 0.8, 3)
-        if np.linalg.norm(target - eye) < 1e-3 * span.max():
-            target = target + span * 0.25
-    size = [(8, 8), (48, 32), (32, 64), (64, 48)][int(rng.integers(0, 4))]
-    bounces = int(rng.integers(0, 14))
-    cf = float(rng.uniform(0.25, 8.0))
-    rr_first = int(rng.integers(1, 6)) if case % 3 == 2 else 0
-    latlong = rng.random() < 0.4
-    return dict(vit=(v, i, pr), eye=tuple(float(x) for x in eye), target=tuple(float(x) for x in target),
-                size=size, bounces=bounces, cf=cf, rr_first=rr_first, latlong=latlong, what=what)
-
-
 @pytest.mark.parametrize("case", range(CASES))
 def test_random_scene_matches_oracle(nb, ctx, case):
-    c = _case(case)
+    c = W.random_case(case)
     v, i, pr = c["vit"]
     blas, tlas, inst, idx = oracle.build_scene(v, i, pr)
     w, h = c["size"]
