@@ -331,6 +331,42 @@ private:
     detail::Handle<nb_target, nb_target_destroy> target_;
 };
 
+// ---- BlitRenderer (src/pathtracing/blit_renderer.rs, blit.wgsl:27-38) ----------------------
+// Draws a Pathtracer's output texture onto an RGBA16F surface of its own size (the surface format
+// of wgpu_context.rs:62): bilinear, clamp-to-edge, surface row 0 at the top.
+class BlitRenderer {
+public:
+    BlitRenderer(const Context &ctx, uint32_t width, uint32_t height) : width_(width), height_(height) {
+        nb_display *d = nullptr;
+        check(nb_display_create(ctx.get(), width, height, &d));
+        h_.reset(d);
+    }
+    // Enqueues the blit of the tracer's output texture (asynchronous on the context stream).
+    void render(const Pathtracer &pathtracer) const { check(nb_display_blit(h_.get(), pathtracer.target())); }
+    // The surface as height x width x 4 half-float bit patterns (waits for the blit).
+    std::vector<uint16_t> surface() const {
+        std::vector<uint16_t> out(static_cast<size_t>(width_) * height_ * 4);
+        check(nb_display_read(h_.get(), out.data()));
+        return out;
+    }
+
+private:
+    uint32_t width_, height_;
+    detail::Handle<nb_display, nb_display_destroy> h_;
+};
+
+// Image files of an output_texture() image (target row order): Portable Float Map (RGB) and
+// OpenEXR (RGBA, FLOAT, or HALF when half is set).
+inline void write_pfm(const std::string &path, const std::vector<float> &rgba, uint32_t width, uint32_t height) {
+    if (rgba.size() != static_cast<size_t>(width) * height * 4) throw Error(NB_ERR_INVALID_ARGUMENT, "image size");
+    check(nb_image_write_pfm(path.c_str(), rgba.data(), width, height));
+}
+inline void write_exr(const std::string &path, const std::vector<float> &rgba, uint32_t width, uint32_t height,
+                      bool half = false) {
+    if (rgba.size() != static_cast<size_t>(width) * height * 4) throw Error(NB_ERR_INVALID_ARGUMENT, "image size");
+    check(nb_image_write_exr(path.c_str(), rgba.data(), width, height, half ? NB_EXR_HALF : 0u));
+}
+
 }  // namespace nbounce
 
 #endif  // NBOUNCE_HPP

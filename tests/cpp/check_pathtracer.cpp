@@ -12,7 +12,9 @@
 //       SceneBuffers::from_scene, EnvMap::load, Pathtracer at resolution factor 1: one
 //       dispatch(), a dispatch_frames() of frames - 2 and one more dispatch(); then with
 //       max_sample_count lowered to the samples taken, one more dispatch() must change nothing.
-//       Writes the RGBA32F image (height x width x 4) to out.raw.
+//       Writes the RGBA32F image (height x width x 4) to out.raw, the BlitRenderer surface of it
+//       at (width + 24) x (height + 8) to out.raw.surface (RGBA16F bits), and the image as PFM and
+//       OpenEXR files to out.raw.pfm and out.raw.exr.
 // Prints one line per check and exits nonzero when one fails.
 #include "nbounce.hpp"
 
@@ -108,6 +110,13 @@ int render_mode(char **argv) {
     pt.dispatch();
     expect(pt.sample_count() == frames && pt.output_texture() == img, "no dispatch beyond max_sample_count");
     write_raw(out, img);
+    // the display and the image files of the same image, beside it
+    const nbounce::BlitRenderer blit(ctx, w + 24, h + 8);
+    blit.render(pt);
+    write_raw(out + ".surface", blit.surface());
+    nbounce::write_pfm(out + ".pfm", img, w, h);
+    nbounce::write_exr(out + ".exr", img, w, h);
+    expect(true, "display and image files written");
     return failures == 0 ? 0 : 1;
 }
 

@@ -2,7 +2,8 @@
 reference's spheres.glb (Scene::parse_gltf, SceneBuffers::from_scene with the reference build),
 a BC6H cube DDS sky (EnvMap::load) and Pathtracer's dispatch / dispatch_frames over 4 frames at
 96x64 give the oracle's image; the program also checks the sample count, the output size and
-that dispatch() does nothing beyond max_sample_count."""
+that dispatch() does nothing beyond max_sample_count.  BlitRenderer's surface of that image
+equals oracle/display.py's blit bit for bit, and write_pfm / write_exr write the image."""
 import os
 import subprocess
 
@@ -12,8 +13,10 @@ import pytest
 import nbounce_amd as nb
 import oracle
 from nbounce_amd import scenegen
+from oracle import display
 from nbounce_amd.pathtracer import DDS_BC6H_UF16, read_dds_cube, write_dds_cube
 from tests import _workloads as W
+from tests.test_image_io import read_exr, read_pfm
 
 pytestmark = pytest.mark.gpu
 
@@ -50,3 +53,11 @@ def test_cpp_pathtracer_matches_oracle(tmp_path):
     err = W.rel_err(got, ref)
     print(f"bit-identical pixels {same:.5f}, max rel err {err.max():.3e}")
     assert err.max() <= 1e-3
+    # BlitRenderer's surface of the image (oracle/display.py's blit, bit for bit) and the files
+    surface = np.fromfile(out + ".surface", dtype=np.uint16).reshape(HEIGHT + 8, WIDTH + 24, 4)
+    want = display.blit(got, WIDTH + 24, HEIGHT + 8)
+    np.testing.assert_array_equal(surface, want.view(np.uint16))
+    np.testing.assert_array_equal(read_pfm(out + ".pfm").view(np.uint32), got[..., :3].view(np.uint32))
+    chans, _ = read_exr(out + ".exr")
+    for k, c in enumerate("RGBA"):
+        np.testing.assert_array_equal(chans[c].view(np.uint32), np.ascontiguousarray(got[::-1, :, k]).view(np.uint32))
