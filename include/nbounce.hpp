@@ -238,7 +238,8 @@ public:
     // Pathtracer::new (pathtracer.rs:51-111): the Sobol table of max_sample_count rows and
     // bounces * 2 + 1 sets (the default 8 bounces: 17, the kernel's row stride) and the output
     // texture of (window * resolution_factor) rounded down to multiples of 8 (:192-202).  The
-    // scene, camera and environment are held by reference and must outlive the tracer.
+    // context, scene and environment are held by reference and must outlive the tracer; the
+    // camera's buffer is copied here and by update().
     Pathtracer(const Context &ctx, const SceneBuffers &scene, const CameraController &camera, const EnvMap &envmap,
                uint32_t window_width, uint32_t window_height, float resolution_factor = 0.3f)
         : resolution_factor(resolution_factor), ctx_(&ctx), scene_(&scene), envmap_(&envmap),
