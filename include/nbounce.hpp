@@ -22,12 +22,10 @@
 
 #include "nbounce.h"
 
-#include <cmath>
 #include <cstdint>
 #include <memory>
 #include <stdexcept>
 #include <string>
-#include <utility>
 #include <vector>
 
 namespace nbounce {
