@@ -385,8 +385,10 @@ nb_status nb_cache_train(nb_cache *cache, const float *raw, const float *target,
  * predicts there, accumulated into the target with globals->weight as nb_render does.  When
  * train_paths > 0, that many full paths of the plain integrator are traced first at a spread of
  * pixels; their shading points (up to three per path) with the path's suffix radiance become a
- * training set for Adam steps of 16384 records with learning rate lr.  *loss (optional) gets the
- * last step's loss.  The cache must have been created on the same context. */
+ * training set for one Adam step over all of them (at most the cache's max_batch records; the
+ * environment variable NB_CACHE_STEP_ROWS=n splits them into steps of n) with learning rate lr.
+ * *loss (optional) gets the last step's loss.  The cache must have been created on the same
+ * context. */
 nb_status nb_render_cached(nb_context *ctx, const nb_scene *scene, nb_target *target, const nb_camera *camera,
                            const nb_globals *globals, const nb_envmap *env, nb_cache *cache, uint32_t train_paths,
                            float lr, float *loss);
