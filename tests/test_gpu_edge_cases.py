@@ -3,7 +3,8 @@
 factor (Russian roulette ends every path after its first scatter, pathtracing.wgsl:227-236),
 16 bounces (the fixed 17-entry Sobol row stride reads two rows further, pathtracing.wgsl:4,151-159),
 the smallest 8x8 target and non-square targets (pathtracer.rs:192-202), and a two-triangle scene
-with one emitter (one instance per triangle, single-leaf BLASes, back-face culling)."""
+with one emitter (one instance per triangle, single-leaf BLASes, back-face culling); and the
+render entry points' argument checks, after which a valid render is unaffected."""
 import numpy as np
 import pytest
 
@@ -147,3 +148,48 @@ def test_two_triangle_scene_matches_oracle(nb, ctx):
     print("case: two-triangle scene 32x32 from behind (both culled: sky only)", flush=True)
     got, ref = _frames_vs_oracle(nb, ctx, sb, osc, env, (32, 32), (0.0, 0.0, -6.0), (0.0, 0.0, 0.0), 2)
     _check(got, ref)
+
+
+def test_render_arguments_are_validated(nb, ctx, small_gpu):
+    """nb_render / nb_render_frames refuse (NB_ERR_INVALID_ARGUMENT, nothing launched) a null
+    scene, objects of another context, a context without a Sobol table and, for
+    nb_render_frames, sample 0; n_frames = 0 renders nothing.  A valid render after all of them
+    gives the image an untouched tracer gives."""
+    import ctypes
+    from nbounce_amd import scenegen
+    from nbounce_amd._lib import lib, ptr
+    w, sb, env = small_gpu
+    P = scenegen.SMALL
+    print("case: invalid render arguments, then a valid render", flush=True)
+    cam = nb.CameraController(world_position=P["eye"], target=P["target"], aspect_ratio=1.0)
+    pt = nb.Pathtracer(ctx, sb, cam, env, 32, 32, resolution_factor=1.0, table=W.table(), sobol_stride=17)
+    camd = cam.data()
+    g = nb.Globals(1, 1.0, 8, 4.0).array()
+    t = pt.output.handle
+    assert lib.nb_render(ctx.handle, None, t, ptr(camd), ptr(g), env.handle) == 1
+    other = nb.Context(0)
+    try:
+        t2 = ctypes.c_void_p()
+        assert lib.nb_target_create(other.handle, 32, 32, ctypes.byref(t2)) == 0
+        # a target of another context, and a context that holds no Sobol table
+        assert lib.nb_render(ctx.handle, sb.handle, t2, ptr(camd), ptr(g), env.handle) == 1
+        assert b"belong to the context" in lib.nb_last_error()
+        sb2 = nb.SceneBuffers(other, w["blas"], w["tlas"], w["instances"], w["vertices"], w["indices"])
+        env2 = nb.EnvMap(other, scenegen.sky_cube(8))
+        assert lib.nb_render(other.handle, sb2.handle, t2, ptr(camd), ptr(g), env2.handle) == 1
+        assert b"Sobol" in lib.nb_last_error()
+        sb2.close()
+        env2.close()
+        lib.nb_target_destroy(t2)
+    finally:
+        other.close()
+    g0 = nb.Globals(0, 0.0, 8, 4.0).array()
+    assert lib.nb_render_frames(ctx.handle, sb.handle, t, ptr(camd), ptr(g0), 2, env.handle) == 1
+    assert lib.nb_render_frames(ctx.handle, sb.handle, t, ptr(camd), ptr(g), 0, env.handle) == 0
+    assert (pt.output_texture() == 0.0).all()
+    pt.dispatch()
+    pt.dispatch()
+    fresh = nb.Pathtracer(ctx, sb, cam, env, 32, 32, resolution_factor=1.0, table=W.table(), sobol_stride=17)
+    fresh.dispatch()
+    fresh.dispatch()
+    np.testing.assert_array_equal(pt.output_texture().view(np.uint32), fresh.output_texture().view(np.uint32))
