@@ -396,16 +396,19 @@ static uint32_t tail_units() {
     return v;
 }
 
-// NB_GRAB_UNITS=n: 16-pixel units per grab before the tail.  Defaults: 4 for a launch alone
-// (finer grabs shorten its tail), 8 for a pipelined frame, whose tail the next frame's paths
-// fill, so coarser grabs (fewer atomics) pay: instanced 1080p, pipelined, 1950 Mrays/s with 4,
-// 1977 with 6, 1986-1988 with 8, 1968 with 12, 1944 with 16.
-static uint32_t grab_units_setting(bool pipelined) {
+// NB_GRAB_UNITS=n: 16-pixel units per grab before the tail.  Defaults: 4 for a one-frame launch
+// alone (finer grabs shorten its tail); 8 for a pipelined frame, whose tail the next frame's
+// paths fill, so coarser grabs (fewer atomics) pay: instanced 1080p, pipelined, 1950 Mrays/s
+// with 4, 1977 with 6, 1986-1988 with 8, 1968 with 12, 1944 with 16; 2 for a launch of several
+// frames, where a unit carries every frame of its pixels (the atomics are amortised and the
+// last units set a long tail): 64 frames 2202-2247 with 2 against 2169-2183 with 4 (1: 2225-
+// 2232), 8 frames 2157-2168 against 2130-2171.
+static uint32_t grab_units_setting(bool pipelined, uint32_t n_frames) {
     static const int v = [] {
         const char *e = std::getenv("NB_GRAB_UNITS");
         return e ? std::atoi(e) : 0;
     }();
-    const int n = v > 0 ? v : (pipelined ? 8 : 4);
+    const int n = v > 0 ? v : (pipelined ? 8 : (n_frames > 1 ? 2 : 4));
     return static_cast<uint32_t>(n > 256 ? 256 : n);
 }
 
@@ -429,7 +432,7 @@ static nb_status launch_regen(const RenderParams &p_in, hipStream_t s) {
     p.regen_min_idle = regen_min_idle();
     p.inner_quorum = inner_quorum();
     p.tail_units = tail_units();
-    p.grab_units = grab_units_setting(p.sample_out != nullptr);
+    p.grab_units = grab_units_setting(p.sample_out != nullptr, p.n_frames);
     const uint32_t grid = p.counters ? regen_grid<K, true>() : regen_grid<K, false>();
     if (grid == 0) return set_error(NB_ERR_HIP, "render: device attribute or occupancy query failed");
     // every wave's first grab_units units of its XCD's band are assigned statically (no atomic
