@@ -403,6 +403,17 @@ nb_status nb_cache_training_records(nb_context *ctx, const nb_scene *scene, uint
                                     const nb_camera *camera, const nb_globals *globals, const nb_envmap *env,
                                     nb_cache *cache, uint32_t train_paths, float *raw, float *target, uint32_t cap,
                                     uint32_t *n);
+/* The cache queries nb_render_cached makes for frame globals->sample of a width x height view,
+ * before any network runs (for inspection and tests; nothing is rendered or trained).  Each
+ * pixel's path runs to its query point: the first shading point reached at bounce >= 1 after a
+ * lobe whose GGX alpha is at least 0.05 (the diffuse lobe counts as 1), or at bounce 3.
+ * direct[4 * pixel] = the radiance met before it (sky or emitter; 0 when queried), alpha 1.  For
+ * each pixel that reached its point, one record: 16 raw floats (as for nb_cache_infer) and
+ * thr[4 * i] = (throughput arriving there, pixel index as float bits).  Records come in no
+ * particular order; *n is the number of queries, of which min(*n, cap) are copied. */
+nb_status nb_cache_query_records(nb_context *ctx, const nb_scene *scene, uint32_t width, uint32_t height,
+                                 const nb_camera *camera, const nb_globals *globals, const nb_envmap *env,
+                                 nb_cache *cache, float *direct, float *raw, float *thr, uint32_t cap, uint32_t *n);
 
 #ifdef __cplusplus
 }

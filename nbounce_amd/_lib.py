@@ -92,6 +92,7 @@ SIGNATURES = {
     "nb_cache_train": [_p, _p, _p, _u32, _u32, _f32, _ptr(_f32)],
     "nb_render_cached": [_p, _p, _p, _p, _p, _p, _p, _u32, _f32, _ptr(_f32)],
     "nb_cache_training_records": [_p, _p, _u32, _u32, _p, _p, _p, _p, _u32, _p, _p, _u32, _ptr(_u32)],
+    "nb_cache_query_records": [_p, _p, _u32, _u32, _p, _p, _p, _p, _p, _p, _p, _u32, _ptr(_u32)],
 }
 
 

@@ -379,20 +379,26 @@ static uint32_t train_sample(uint32_t sample) { return 1u + (sample + 511u) % 10
 
 // One cache-assisted frame into a float4 target of width x height pixels on the context's stream
 // (nb_render_cached's work; also used by the server for cached requests).
+// The per-pixel workspace of cache frames (direct radiance, query records, predictions) for at
+// least npix pixels.
+static nb_status ensure_workspace(nb_context *ctx, nb_cache *c, size_t npix) {
+    if (npix <= c->ws_pixels) return NB_OK;
+    C_HIP(hipStreamSynchronize(ctx->stream));
+    hipFree(c->direct); hipFree(c->q_raw); hipFree(c->q_thr); hipFree(c->q_rgb);
+    c->direct = nullptr; c->q_raw = c->q_thr = c->q_rgb = nullptr; c->ws_pixels = 0;
+    C_HIP(hipMalloc(&c->direct, npix * 16));
+    C_HIP(hipMalloc(&c->q_raw, npix * kMlpRaw * 4));
+    C_HIP(hipMalloc(&c->q_thr, npix * 16));
+    C_HIP(hipMalloc(&c->q_rgb, npix * 12));
+    c->ws_pixels = npix;
+    return NB_OK;
+}
+
 nb_status cache_render_frame(nb_context *ctx, const DevScene &scene, const DevEnv &env, float4 *target, uint32_t width,
                              uint32_t height, const float *c2w, const nb_globals &g, nb_cache *c, uint32_t train_paths,
                              float lr, float *loss) {
     const size_t npix = static_cast<size_t>(width) * height;
-    if (npix > c->ws_pixels) {
-        C_HIP(hipStreamSynchronize(ctx->stream));
-        hipFree(c->direct); hipFree(c->q_raw); hipFree(c->q_thr); hipFree(c->q_rgb);
-        c->direct = nullptr; c->q_raw = c->q_thr = c->q_rgb = nullptr; c->ws_pixels = 0;
-        C_HIP(hipMalloc(&c->direct, npix * 16));
-        C_HIP(hipMalloc(&c->q_raw, npix * kMlpRaw * 4));
-        C_HIP(hipMalloc(&c->q_thr, npix * 16));
-        C_HIP(hipMalloc(&c->q_rgb, npix * 12));
-        c->ws_pixels = npix;
-    }
+    C_TRY(ensure_workspace(ctx, c, npix));
     CacheFrame f{};
     f.scene = scene;
     f.env = env;
@@ -512,6 +518,58 @@ nb_status nb_cache_training_records(nb_context *ctx, const nb_scene *scene, uint
     if (k) {
         C_HIP(hipMemcpy(raw, c->raw, static_cast<size_t>(k) * kMlpRaw * 4, hipMemcpyDeviceToHost));
         C_HIP(hipMemcpy(target, c->target, static_cast<size_t>(k) * 12, hipMemcpyDeviceToHost));
+    }
+    *n = count;
+    return NB_OK;
+}
+
+nb_status nb_cache_query_records(nb_context *ctx, const nb_scene *scene, uint32_t width, uint32_t height,
+                                 const nb_camera *camera, const nb_globals *g, const nb_envmap *env, nb_cache *c,
+                                 float *direct, float *raw, float *thr, uint32_t cap, uint32_t *n) {
+    if (!ctx || !scene || !camera || !g || !env || !c || !n || !direct || (cap && (!raw || !thr)))
+        return set_error(NB_ERR_INVALID_ARGUMENT, "null argument");
+    if (scene->ctx != ctx || env->ctx != ctx || c->ctx != ctx)
+        return set_error(NB_ERR_INVALID_ARGUMENT, "all objects must belong to the context");
+    if (!ctx->sobol) return set_error(NB_ERR_INVALID_ARGUMENT, "no Sobol table set (nb_sobol_set)");
+    if (g->sample == 0) return set_error(NB_ERR_INVALID_ARGUMENT, "samples are 1-based");
+    if (!width || !height || width % 8 || height % 8)
+        return set_error(NB_ERR_INVALID_ARGUMENT, "need a nonzero multiple-of-8 size");
+    if (g->bounces > kMaxBounceLimit)
+        return set_error(NB_ERR_INVALID_ARGUMENT, "bounces %u exceeds %u", g->bounces, kMaxBounceLimit);
+    C_HIP(hipSetDevice(ctx->device));
+    const size_t npix = static_cast<size_t>(width) * height;
+    C_TRY(ensure_workspace(ctx, c, npix));
+    // the query part of cache_render_frame's frame, as nb_render_cached launches it
+    CacheFrame f{};
+    f.scene = scene->dev;
+    f.env = dev_env(env);
+    f.sobol = ctx->sobol;
+    f.sobol_entries = ctx->sobol_entries;
+    f.sobol_stride = ctx->sobol_stride;
+    std::memcpy(f.c2w, camera->clip_to_world, sizeof(f.c2w));
+    f.width = width;
+    f.height = height;
+    f.sample = g->sample;
+    f.bounces = g->bounces;
+    f.contribution_factor = g->contribution_factor;
+    f.mirror_alpha = 0.05f;
+    f.max_cache_bounce = 3;
+    f.direct = c->direct;
+    f.q_raw = c->q_raw;
+    f.q_thr = c->q_thr;
+    f.q_count = c->counts;
+    f.tiles_x = width / 8;
+    f.n_tiles = (width / 8) * (height / 8);
+    f.tiles_per_xcd = (f.n_tiles + 7) / 8;
+    C_TRY(cache_launch_query(f, ctx->stream));
+    uint32_t count = 0;
+    C_HIP(hipMemcpyAsync(&count, c->counts, 4, hipMemcpyDeviceToHost, ctx->stream));
+    C_HIP(hipStreamSynchronize(ctx->stream));
+    C_HIP(hipMemcpy(direct, c->direct, npix * 16, hipMemcpyDeviceToHost));
+    const uint32_t k = count < cap ? count : cap;
+    if (k) {
+        C_HIP(hipMemcpy(raw, c->q_raw, static_cast<size_t>(k) * kMlpRaw * 4, hipMemcpyDeviceToHost));
+        C_HIP(hipMemcpy(thr, c->q_thr, static_cast<size_t>(k) * 16, hipMemcpyDeviceToHost));
     }
     *n = count;
     return NB_OK;

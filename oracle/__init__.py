@@ -71,6 +71,9 @@ def _load():
         [ctypes.c_uint32] * 4 + [ctypes.c_void_p, ctypes.c_void_p]
     lib.orc_train_path.argtypes = [ctypes.c_void_p] * 4 + [ctypes.c_uint32] * 2 + [ctypes.c_void_p] + \
         [ctypes.c_uint32] * 4 + [ctypes.c_void_p] * 4
+    lib.orc_cache_query.restype = ctypes.c_uint32
+    lib.orc_cache_query.argtypes = [ctypes.c_void_p] * 4 + [ctypes.c_uint32] * 2 + [ctypes.c_void_p] + \
+        [ctypes.c_uint32] * 4 + [ctypes.c_float, ctypes.c_uint32] + [ctypes.c_void_p] * 3
     lib.orc_primary_hits.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32,
                                      ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_void_p]
     lib.orc_build_scene.argtypes = [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_void_p, ctypes.c_uint32,
@@ -292,6 +295,19 @@ def train_path(scene, cam, g, table, stride, env, width, height, x, y):
     lib.orc_train_path(scene.ref, p(cam), p(g), p(t), stride, t.shape[0], env.ref, width, height, x, y, p(raw),
                        p(target), ctypes.byref(n), p(rad))
     return raw[:n.value], target[:n.value], rad
+
+
+def cache_query(scene, cam, g, table, stride, env, width, height, x, y, mirror_alpha=0.05, max_cache_bounce=3):
+    """The learned cache's query of pixel (x, y) (orc_cache_query): returns (queried, direct (3,),
+    raw (16,), throughput (3,)); raw and throughput are zero when the path did not reach its
+    query point."""
+    t = np.ascontiguousarray(table, dtype=np.float32)
+    direct = np.zeros(3, dtype=np.float32)
+    raw = np.zeros(16, dtype=np.float32)
+    thr = np.zeros(3, dtype=np.float32)
+    q = lib.orc_cache_query(scene.ref, p(cam), p(g), p(t), stride, t.shape[0], env.ref, width, height, x, y,
+                            mirror_alpha, max_cache_bounce, p(direct), p(raw), p(thr))
+    return bool(q), direct, raw, thr
 
 
 def pixel_shift(x, y):

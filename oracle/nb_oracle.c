@@ -1081,13 +1081,23 @@ void orc_set_roulette_first(uint32_t bounce) { g_roulette_first = bounce; }
 /* Shading points of a path as the learned cache's training records see them (no counterpart in
  * the reference): up to max_rec non-emissive hits, each as 16 floats (position, outgoing
  * direction, normal, roughness, albedo, metallic, 0, 0) with the throughput that arrived there. */
-typedef struct { float *raw; v3 *thr; uint32_t n, max; } path_rec;
+/* Shading-point records of a path: the first `max` non-emissive shading points (training
+ * records), or, with query set, the learned cache's query point alone: the first shading point
+ * reached at bounce >= 1 after a lobe whose alpha is at least mirror_alpha, or at bounce
+ * max_cache_bounce, where the path then stops (queried = 1) without shading it. */
+typedef struct {
+    float *raw; v3 *thr; uint32_t n, max;
+    int query; float mirror_alpha; uint32_t max_cache_bounce; int queried;
+} path_rec;
 
 static v3 sample_rendering_eq_rec(const orc_scene *sc, const orc_globals *g, const float *table, uint32_t stride,
                                   uint32_t table_entries, const orc_env *env, const float shift[4], ray_t ray,
                                   orc_counters *cnt, path_rec *rec) {
     v3 thr = mk3(1.0f, 1.0f, 1.0f);
+    float prev_alpha = 0.0f;   /* alpha of the previous bounce's lobe (1 for the diffuse lobe) */
     for (uint32_t bounce = 0; bounce <= g->bounces; ++bounce) {
+        const int stop = rec && rec->query && bounce >= 1 &&
+                         (prev_alpha >= rec->mirror_alpha || bounce >= rec->max_cache_bounce);
         hit_info hit = intersect_scene(sc, &ray, cnt);
         if (hit.dist == NO_HIT) {
             float d[3] = {ray.d.x, ray.d.y, ray.d.z}, e[3];
@@ -1098,7 +1108,7 @@ static v3 sample_rendering_eq_rec(const orc_scene *sc, const orc_globals *g, con
         float alpha = hit.roughness * hit.roughness;
         float alpha2 = alpha * alpha;
         v3 n = normalize3(hit.normal);
-        if (rec && rec->n < rec->max) {
+        if (rec && rec->n < rec->max && (!rec->query || stop)) {
             v3 wo_ = normalize3(mk3(-ray.d.x, -ray.d.y, -ray.d.z));
             float *r = rec->raw + 16u * rec->n;
             r[0] = hit.position.x; r[1] = hit.position.y; r[2] = hit.position.z;
@@ -1110,6 +1120,7 @@ static v3 sample_rendering_eq_rec(const orc_scene *sc, const orc_globals *g, con
             rec->thr[rec->n] = thr;
             rec->n += 1;
         }
+        if (stop) { rec->queried = 1; return mk3(0.0f, 0.0f, 0.0f); }
         float s0[4], s1[4];
         uint32_t e0 = g->sample * stride + 1u + bounce * 2u;
         sobol_fetch(table, table_entries, e0, shift, s0);
@@ -1140,6 +1151,7 @@ static v3 sample_rendering_eq_rec(const orc_scene *sc, const orc_globals *g, con
             float den = (1.0f + ll) + lv;
             v3 spec = mk3((F.x * (1.0f + lv)) / den, (F.y * (1.0f + lv)) / den, (F.z * (1.0f + lv)) / den);
             thr = mul3(thr, divs3(spec, p_spec));
+            prev_alpha = alpha;
         } else {
             v3 t = hit.tangent;
             v3 nt = normalize3(sub3(t, scl3(dot3(t, n), n)));
@@ -1154,6 +1166,7 @@ static v3 sample_rendering_eq_rec(const orc_scene *sc, const orc_globals *g, con
             float om = 1.0f - metallic;
             v3 diff = mk3((om * albedo.x) * resp, (om * albedo.y) * resp, (om * albedo.z) * resp);
             thr = mul3(thr, divs3(diff, p_diff));
+            prev_alpha = 1.0f;
         }
         if (bounce >= g_roulette_first) {   /* the reference: from bounce 0 */
             float p_cont = fmin_(1.0f - pow8f((float)bounce / (float)g->bounces), 1.0f);
@@ -1183,7 +1196,7 @@ void orc_train_path(const orc_scene *sc, const orc_camera *cam, const orc_global
     sobol_fetch(table, rows * stride, g->sample * stride, shift, jit);
     ray_t ray = generate_ray(cam, x, y, w, h, jit[0], jit[1]);
     v3 thr[3];
-    path_rec rec = {raw, thr, 0u, 3u};
+    path_rec rec = {raw, thr, 0u, 3u, 0, 0.0f, 0u, 0};
     orc_counters cnt = {0, 0, 0, 0};
     v3 out = sample_rendering_eq_rec(sc, g, table, stride, rows * stride, env, shift, ray, &cnt, &rec);
     for (uint32_t k = 0; k < rec.n; ++k) {
@@ -1193,6 +1206,28 @@ void orc_train_path(const orc_scene *sc, const orc_camera *cam, const orc_global
     }
     *n_rec = rec.n;
     radiance[0] = out.x; radiance[1] = out.y; radiance[2] = out.z;
+}
+
+/* The learned cache's query for pixel (x, y) of frame g->sample: the same path as
+ * orc_sample_pixel up to its query point.  Returns 1 with the point's record (raw[16]) and the
+ * throughput arriving there (thr[3]) when the path reaches it; direct[3] = the radiance met
+ * before (sky, emitter), 0 when queried. */
+uint32_t orc_cache_query(const orc_scene *sc, const orc_camera *cam, const orc_globals *g, const float *table,
+                         uint32_t stride, uint32_t rows, const orc_env *env, uint32_t w, uint32_t h, uint32_t x,
+                         uint32_t y, float mirror_alpha, uint32_t max_cache_bounce, float direct[3], float raw[16],
+                         float thr_out[3]) {
+    float shift[4];
+    pixel_shift(x, y, shift);
+    float jit[4];
+    sobol_fetch(table, rows * stride, g->sample * stride, shift, jit);
+    ray_t ray = generate_ray(cam, x, y, w, h, jit[0], jit[1]);
+    v3 thr[1];
+    path_rec rec = {raw, thr, 0u, 1u, 1, mirror_alpha, max_cache_bounce, 0};
+    orc_counters cnt = {0, 0, 0, 0};
+    v3 out = sample_rendering_eq_rec(sc, g, table, stride, rows * stride, env, shift, ray, &cnt, &rec);
+    direct[0] = out.x; direct[1] = out.y; direct[2] = out.z;
+    if (rec.queried) { thr_out[0] = thr[0].x; thr_out[1] = thr[0].y; thr_out[2] = thr[0].z; }
+    return (uint32_t)rec.queried;
 }
 
 void orc_sample_pixel(const orc_scene *sc, const orc_camera *cam, const orc_globals *g, const float *table,

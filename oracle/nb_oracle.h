@@ -36,6 +36,11 @@
  *                             lat-long form (no reference): constant field, linear-in-u rows,
  *                             one-hot texels at +x / +z / -z / +y, seam continuity, the baked
  *                             sky against the cube bake of the same sky
+ *   orc_cache_query           (no reference: the learned cache's query point) the same path as
+ *                             orc_sample_pixel (equal radiance bits where it ends before the
+ *                             point); the record equals orc_train_path's k-th record, k = 1 for
+ *                             rough materials, max_cache_bounce for smooth metal, 1 again with a
+ *                             lower mirror_alpha (tests/test_oracle_cache_query.py)
  *   orc_set_roulette_first    (no reference: nb_render_options) forced roulette numbers give a
  *                             path of exactly first + 1 rays; 0 leaves the reference's rule
  *   orc_render                pathtracing.wgsl:27-272    white-furnace / closed-form single-bounce
@@ -175,6 +180,17 @@ void orc_train_path(const orc_scene *scene, const orc_camera *cam, const orc_glo
                     const float *sobol_table, uint32_t sobol_stride, uint32_t sobol_rows,
                     const orc_env *env, uint32_t width, uint32_t height, uint32_t x, uint32_t y,
                     float raw[48], float target[9], uint32_t *n_rec, float radiance[3]);
+/* The learned cache's query for pixel (x, y) of frame g->sample (no counterpart in the
+ * reference): the path of orc_sample_pixel up to the first shading point reached at bounce >= 1
+ * after a lobe whose GGX alpha is at least mirror_alpha (the diffuse lobe counts as 1), or at
+ * bounce max_cache_bounce.  Returns 1 when the path reaches that point, with its record in
+ * raw[16] (as for orc_train_path) and the throughput arriving there in thr[3]; direct[3] is the
+ * radiance met before it (sky or emitter; 0 when queried or ended by roulette or the limit). */
+uint32_t orc_cache_query(const orc_scene *scene, const orc_camera *cam, const orc_globals *g,
+                         const float *sobol_table, uint32_t sobol_stride, uint32_t sobol_rows,
+                         const orc_env *env, uint32_t width, uint32_t height, uint32_t x, uint32_t y,
+                         float mirror_alpha, uint32_t max_cache_bounce, float direct[3], float raw[16],
+                         float thr[3]);
 /* Same integrator but sums n_samples samples per pixel (rows first..first+n-1) in float64:
  * out64[p*3+c] = mean radiance. Used as the high-spp reference image. */
 void orc_render_mean_f64(const orc_scene *scene, const orc_camera *cam, const orc_globals *g,

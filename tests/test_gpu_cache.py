@@ -285,3 +285,59 @@ def test_training_records_match_oracle():
     np.testing.assert_array_equal(canon(got), canon(want))
     lib.nb_cache_destroy(h)
     ctx.close()
+
+
+def test_query_records_match_oracle():
+    """The queries of a cache-assisted frame (nb_cache_query_records: each pixel's path up to its
+    query point, the radiance met before it, the point's record and the throughput arriving
+    there) equal the CPU oracle's restatement of the same paths (orc_cache_query), pixel by pixel
+    and bit for bit: which pixels query, their records and throughputs, and every pixel's direct
+    radiance."""
+    import nbounce_amd as nb
+    import oracle
+    from nbounce_amd import scenegen
+    from nbounce_amd._lib import check, lib, ptr
+    from tests import _workloads as Wl
+    print("case: the queries of frame 3, 96x64 spheres scene, 8 bounces", flush=True)
+    w_, h_, sample, bounces, cf = 96, 64, 3, 8, 4.0
+    ctx = nb.Context(0)
+    wl = Wl.small()
+    sb = nb.SceneBuffers(ctx, wl["blas"], wl["tlas"], wl["instances"], wl["vertices"], wl["indices"])
+    faces = scenegen.sky_cube(64)
+    env = nb.EnvMap(ctx, faces)
+    table = Wl.table()
+    check(lib.nb_sobol_set(ctx.handle, ptr(np.ascontiguousarray(table)), table.shape[0], table.shape[1], 17))
+    eye, target = scenegen.orbit_pose(0)
+    cam = nb.CameraController(world_position=eye, target=target, aspect_ratio=w_ / h_).data()
+    h = ctypes.c_void_p()
+    check(lib.nb_cache_create_ex(ctx.handle, ptr(BOX_MIN), ptr(BOX_MAX), 3, 1 << 16, 0, ctypes.byref(h)))
+    g = nb.Globals(sample, 1.0, bounces, cf).array()
+    npix = w_ * h_
+    direct = np.zeros((h_, w_, 4), dtype=np.float32)
+    raw = np.zeros((npix, 16), dtype=np.float32)
+    thr = np.zeros((npix, 4), dtype=np.float32)
+    n = ctypes.c_uint32()
+    check(lib.nb_cache_query_records(ctx.handle, sb.handle, w_, h_, ptr(cam), ptr(g), env.handle, h, ptr(direct),
+                                     ptr(raw), ptr(thr), npix, ctypes.byref(n)))
+    pix = thr[:n.value, 3].view(np.uint32)
+    assert len(set(pix.tolist())) == n.value
+    got_raw = {int(p): raw[i] for i, p in enumerate(pix)}
+    got_thr = {int(p): thr[i, :3] for i, p in enumerate(pix)}
+    osc, oenv = Wl.oracle_scene(wl), oracle.Env(faces)
+    og = oracle.globals_(sample, 1.0, bounces, cf)
+    queried = 0
+    for y in range(h_):
+        for x in range(w_):
+            p = y * w_ + x
+            q, d, r, t = oracle.cache_query(osc, cam, og, table, 17, oenv, w_, h_, x, y)
+            assert (p in got_raw) == q, (x, y)
+            np.testing.assert_array_equal(direct[y, x, :3].view(np.uint32), d.view(np.uint32))
+            if q:
+                queried += 1
+                np.testing.assert_array_equal(got_raw[p].view(np.uint32), r.view(np.uint32))
+                np.testing.assert_array_equal(got_thr[p].view(np.uint32), t.view(np.uint32))
+    assert (direct[..., 3] == 1.0).all()
+    print(f"queries: {queried} of {npix} pixels", flush=True)
+    assert queried == n.value and 0 < queried < npix
+    lib.nb_cache_destroy(h)
+    ctx.close()
