@@ -341,3 +341,53 @@ def test_query_records_match_oracle():
     assert queried == n.value and 0 < queried < npix
     lib.nb_cache_destroy(h)
     ctx.close()
+
+
+def test_cached_frame_is_built_from_its_queries():
+    """A cache-assisted frame's image (nb_render_cached, inference only, weight 1) is exactly
+    direct + throughput * max(prediction, 0) of its queries: the query records
+    (nb_cache_query_records, pinned to the oracle above) run through the network
+    (nb_cache_infer, pinned to the oracle above) and combined in binary32 in the kernel's order."""
+    import nbounce_amd as nb
+    from nbounce_amd import scenegen
+    from nbounce_amd._lib import check, lib, ptr
+    from tests import _workloads as Wl
+    print("case: a cache-assisted frame rebuilt from its queries, 96x64 spheres scene", flush=True)
+    w_, h_, sample, bounces, cf = 96, 64, 4, 8, 4.0
+    ctx = nb.Context(0)
+    wl = Wl.small()
+    sb = nb.SceneBuffers(ctx, wl["blas"], wl["tlas"], wl["instances"], wl["vertices"], wl["indices"])
+    env = nb.EnvMap(ctx, scenegen.sky_cube(64))
+    table = Wl.table()
+    check(lib.nb_sobol_set(ctx.handle, ptr(np.ascontiguousarray(table)), table.shape[0], table.shape[1], 17))
+    eye, target = scenegen.orbit_pose(7)
+    cam = nb.CameraController(world_position=eye, target=target, aspect_ratio=w_ / h_).data()
+    h = ctypes.c_void_p()
+    check(lib.nb_cache_create_ex(ctx.handle, ptr(BOX_MIN), ptr(BOX_MAX), 3, 1 << 16, 0, ctypes.byref(h)))
+    params = mlp.init_params(11)
+    params[np.arange(len(params)) % 7 == 0] += 0.05      # nonzero predictions
+    check(lib.nb_cache_set_params(h, ptr(params)))
+    g = nb.Globals(sample, 1.0, bounces, cf).array()
+    t = ctypes.c_void_p()
+    check(lib.nb_target_create(ctx.handle, w_, h_, ctypes.byref(t)))
+    check(lib.nb_render_cached(ctx.handle, sb.handle, t, ptr(cam), ptr(g), env.handle, h, 0, 0.0, None))
+    img = np.zeros((h_, w_, 4), dtype=np.float32)
+    check(lib.nb_target_read(t, ptr(img)))
+    npix = w_ * h_
+    direct = np.zeros((h_, w_, 4), dtype=np.float32)
+    raw = np.zeros((npix, 16), dtype=np.float32)
+    thr = np.zeros((npix, 4), dtype=np.float32)
+    n = ctypes.c_uint32()
+    check(lib.nb_cache_query_records(ctx.handle, sb.handle, w_, h_, ptr(cam), ptr(g), env.handle, h, ptr(direct),
+                                     ptr(raw), ptr(thr), npix, ctypes.byref(n)))
+    k = n.value
+    rgb = np.zeros((k, 3), dtype=np.float32)
+    check(lib.nb_cache_infer(h, ptr(np.ascontiguousarray(raw[:k])), k, ptr(rgb)))
+    want = direct.reshape(npix, 4)[:, :3].copy()
+    pix = thr[:k, 3].view(np.uint32)
+    want[pix] = want[pix] + thr[:k, :3] * np.maximum(rgb, np.float32(0.0))
+    assert (rgb > 0).mean() > 0.2 and k > 0.2 * npix
+    np.testing.assert_array_equal(img.reshape(npix, 4)[:, :3].view(np.uint32), want.view(np.uint32))
+    lib.nb_target_destroy(t)
+    lib.nb_cache_destroy(h)
+    ctx.close()
