@@ -124,3 +124,16 @@ def init_params(seed=0):
         ps.append(rng.uniform(-bound, bound, size=o * i))
         ps.append(np.zeros(o))
     return np.concatenate(ps).astype(np.float32)
+
+
+def adam_step(w, m, v, g, t, lr, b1=0.9, b2=0.99, eps=1e-8):
+    """One Adam step (Kingma & Ba 2015, Algorithm 1) in float64 on the flat parameters: step t
+    (1-based) with gradient g; returns the new (w, m, v).  The update is
+    lr * m_hat / (sqrt(v_hat) + eps) with m_hat = m / (1 - b1^t), v_hat = v / (1 - b2^t); the
+    cache trains with b1 = 0.9, b2 = 0.99, eps = 1e-8 (cache.cpp)."""
+    w, m, v, g = (np.asarray(a, dtype=np.float64) for a in (w, m, v, g))
+    m = b1 * m + (1.0 - b1) * g
+    v = b2 * v + (1.0 - b2) * g * g
+    m_hat = m / (1.0 - b1 ** t)
+    v_hat = v / (1.0 - b2 ** t)
+    return w - lr * m_hat / (np.sqrt(v_hat) + eps), m, v

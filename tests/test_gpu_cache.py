@@ -391,3 +391,46 @@ def test_cached_frame_is_built_from_its_queries():
     lib.nb_target_destroy(t)
     lib.nb_cache_destroy(h)
     ctx.close()
+
+
+def test_adam_steps_match_oracle():
+    """Three Adam steps of a BF16 cache (nb_cache_train with one batch per call, so one step
+    each): every parameter's update equals the oracle's adam_step (oracle/mlp.py, pinned to
+    torch.optim.Adam) applied to the GPU's own gradient at the same parameters
+    (nb_cache_gradient).  float32 arithmetic and the gradient atomics' summation order are the
+    differences, hence a relative 1e-3 on the update (plus two units in the last place of the
+    float32 parameter it is read back from), where |g| > 1e-6."""
+    import nbounce_amd as nb
+    from nbounce_amd._lib import check, lib, ptr
+    print("case: 3 Adam steps of a BF16 cache on 4096 records", flush=True)
+    ctx = nb.Context(0)
+    h = ctypes.c_void_p()
+    check(lib.nb_cache_create_ex(ctx.handle, ptr(BOX_MIN), ptr(BOX_MAX), 5, 1 << 16, 0, ctypes.byref(h)))
+    raw, target = _batch(np.random.default_rng(21), 4096)
+    params = mlp.init_params(13)
+    params[np.arange(len(params)) % 5 == 0] += 0.02
+    check(lib.nb_cache_set_params(h, ptr(params)))
+    m = np.zeros(len(params))
+    v = np.zeros(len(params))
+    loss = ctypes.c_float()
+    for t in range(1, 4):
+        before = np.zeros(lib.nb_cache_param_count(), dtype=np.float32)
+        check(lib.nb_cache_get_params(h, ptr(before)))
+        g = np.zeros(len(params), dtype=np.float32)
+        check(lib.nb_cache_gradient(h, ptr(raw), ptr(target), 4096, ctypes.byref(loss), ptr(g)))
+        check(lib.nb_cache_train(h, ptr(raw), ptr(target), 4096, 4096, 1e-3, ctypes.byref(loss)))
+        after = np.zeros(len(params), dtype=np.float32)
+        check(lib.nb_cache_get_params(h, ptr(after)))
+        w, m, v = mlp.adam_step(before, m, v, g, t, 1e-3)
+        mask = np.abs(g) > 1e-6
+        upd_gpu = after.astype(np.float64) - before
+        upd_or = w - before
+        print(f"step {t}: {mask.sum()} parameters compared, max |update| {np.abs(upd_gpu).max():.3e}", flush=True)
+        assert mask.mean() > 0.5
+        # the updates are read back as differences of float32 parameters: two units in the last
+        # place of the parameter on top of the relative bound
+        tol = 1e-3 * np.abs(upd_or) + 2.0 * np.spacing(np.abs(before).astype(np.float32)).astype(np.float64)
+        bad = np.abs(upd_gpu - upd_or) > tol
+        assert not bad[mask].any(), (np.flatnonzero(mask & bad)[:5], upd_gpu[mask & bad][:5], upd_or[mask & bad][:5])
+    lib.nb_cache_destroy(h)
+    ctx.close()

@@ -109,3 +109,28 @@ def test_loss_scale_is_divided_back_out():
     _, g64, _ = mlp.loss_and_grad(p, x, near, fmt="fp16", scale=2.0 ** 20)
     assert np.count_nonzero(g16u) < 0.5 * np.count_nonzero(g16)
     assert np.linalg.norm(g16 - g64) < 0.1 * np.linalg.norm(g64)
+
+
+def test_adam_step_matches_torch_and_the_constant_gradient_closed_form():
+    """oracle/mlp.py's adam_step against torch.optim.Adam (same betas and eps, float64) over six
+    steps of random gradients, and against Adam's closed form for a constant gradient: every
+    step moves each weight by lr * g / (|g| + eps'), i.e. by lr * sign(g) for |g| >> eps (a
+    missing or swapped bias correction breaks this from step 2 on, a sign slip at once)."""
+    import torch
+    rng = np.random.default_rng(9)
+    w0 = rng.normal(size=200)
+    grads = [rng.normal(scale=10.0 ** rng.uniform(-3, 1), size=200) for _ in range(6)]
+    p = torch.nn.Parameter(torch.tensor(w0, dtype=torch.float64))
+    opt = torch.optim.Adam([p], lr=1e-3, betas=(0.9, 0.99), eps=1e-8)
+    w, m, v = w0.copy(), np.zeros(200), np.zeros(200)
+    for t, g in enumerate(grads, start=1):
+        p.grad = torch.tensor(g, dtype=torch.float64)
+        opt.step()
+        w, m, v = mlp.adam_step(w, m, v, g, t, 1e-3)
+        np.testing.assert_allclose(w, p.detach().numpy(), rtol=0, atol=1e-14)
+    g = rng.choice([-1.0, 1.0], size=200) * rng.uniform(0.01, 5.0, size=200)
+    w, m, v = w0.copy(), np.zeros(200), np.zeros(200)
+    for t in range(1, 6):
+        prev = w
+        w, m, v = mlp.adam_step(w, m, v, g, t, 1e-3)
+        np.testing.assert_allclose(w - prev, -1e-3 * np.sign(g), rtol=1e-5, atol=0)
