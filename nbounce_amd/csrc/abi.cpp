@@ -550,7 +550,7 @@ nb_status render_frames_sample_parallel(nb_context *ctx, const nb_scene *scene, 
         st.grab_units = 4;
         NB_HIP(hipMemcpyAsync(d, &h, sizeof(SpDesc), hipMemcpyHostToDevice, ctx->stream));
         NB_TRY(serve_launch_reset(&d->step, ctx->stream));
-        NB_TRY(serve_launch_step(&d->step, ctx->stream));
+        NB_TRY(serve_launch_step(&d->step, ctx->stream, 4u));
         int cus = 0;
         NB_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ctx->device));
         NB_TRY(serve_launch_accumulate(&d->step, static_cast<uint32_t>(cus > 0 ? cus : 64) * 8u, ctx->stream));

@@ -170,7 +170,9 @@ constexpr uint32_t kServeMaxBounces = 32;
 constexpr uint32_t kServeMaxGroups = 64;   // one per resident scene at most
 
 nb_status serve_launch_reset(const ServeStep *step, hipStream_t s);
-nb_status serve_launch_step(const ServeStep *step, hipStream_t s);
+// waves_per_simd: 5 for the server's steps, 4 for the sample-parallel nb_render_frames (the
+// occupancy each measured best at).
+nb_status serve_launch_step(const ServeStep *step, hipStream_t s, uint32_t waves_per_simd = 5);
 nb_status serve_launch_accumulate(const ServeStep *step, uint32_t grid, hipStream_t s);
 
 // Kernel launchers (trace.hip).

@@ -75,11 +75,13 @@ __device__ __forceinline__ DevScene uniform_scene(const DevScene &d) {
     return u;
 }
 
-// Held to 5 waves per SIMD (96 VGPRs, 16 dwords spilled): over the serving stream the steps'
-// kernel time is 3 % lower than at 4 waves (110 VGPRs, no spills).  The renderer's kernel, the
-// same traversal, loses 11 % at 5 waves.
-template <int K>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5))) void serve_pt_kernel(const ServeStep *step) {
+// Held to kWaves waves per SIMD.  The server runs 5 (96 VGPRs, 31 dwords spilled): over the
+// serving stream (20 QPS, 200 mixed requests) the steps' kernel time is 1393-1395 ms against
+// 1401 ms at 4 waves.  The sample-parallel nb_render_frames runs 4 (no spills): 256x256 x 16
+// spp 2183-2186 Mrays/s against 2107 at 5 waves.  The renderer's kernel, the same traversal,
+// loses 11 % at 5 waves.
+template <int K, int kWaves>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(kWaves))) void serve_pt_kernel(const ServeStep *step) {
     __shared__ unsigned long long s_stack[K * 64];
     const ServeStep &S = *step;
     if (S.n_units == 0u || S.n_groups == 0u) return;
@@ -270,12 +272,13 @@ nb_status check_launch(const char *what) {
 }
 
 // Resident waves of serve_pt_kernel on the device (occupancy x CUs), a multiple of 8.
+template <int kWaves>
 uint32_t serve_grid() {
     static const uint32_t grid = [] {
         int dev = 0, cus = 0, per_cu = 0;
         if (hipGetDevice(&dev) != hipSuccess ||
             hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
-            hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, serve_pt_kernel<kServeK>, 64, 0) != hipSuccess)
+            hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, serve_pt_kernel<kServeK, kWaves>, 64, 0) != hipSuccess)
             return 0u;
         const uint32_t g = static_cast<uint32_t>(cus) * static_cast<uint32_t>(per_cu < 1 ? 1 : per_cu) / 8u * 8u;
         return g < 8u ? 8u : g;
@@ -295,10 +298,11 @@ nb_status serve_launch_accumulate(const ServeStep *step, uint32_t grid, hipStrea
     return check_launch("serve accumulate");
 }
 
-nb_status serve_launch_step(const ServeStep *step, hipStream_t s) {
-    const uint32_t grid = serve_grid();
+nb_status serve_launch_step(const ServeStep *step, hipStream_t s, uint32_t waves_per_simd) {
+    const uint32_t grid = waves_per_simd == 4u ? serve_grid<4>() : serve_grid<5>();
     if (grid == 0u) return set_error(NB_ERR_HIP, "serve kernel occupancy query failed");
-    hipLaunchKernelGGL((serve_pt_kernel<kServeK>), dim3(grid), dim3(64), 0, s, step);
+    if (waves_per_simd == 4u) hipLaunchKernelGGL((serve_pt_kernel<kServeK, 4>), dim3(grid), dim3(64), 0, s, step);
+    else hipLaunchKernelGGL((serve_pt_kernel<kServeK, 5>), dim3(grid), dim3(64), 0, s, step);
     return check_launch("serve step");
 }
 
