@@ -100,6 +100,9 @@ nb_status nb_context_synchronize(nb_context *ctx);
  * that has a mesh, with the node's local matrix (parents ignored, as in the reference).  A
  * primitive that is not a triangle list or lacks POSITION, NORMAL, TANGENT or indices fails
  * with NB_ERR_FORMAT and the reference's MeshError text; the scene is then left unchanged.
+ * Accessor and view bounds are checked before anything is allocated, and a sparse accessor
+ * without a bufferView may take at most 256 MiB, so a small file cannot make the reader claim
+ * much more memory than it holds.
  * flags: NB_GLTF_SHARE_GEOMETRY appends the geometry of a mesh primitive once per distinct
  * accessor set, so primitives that reuse accessors share one index range (pair it with
  * NB_BUILD_SHARE_BLAS); the reference appends a copy per mesh primitive. */

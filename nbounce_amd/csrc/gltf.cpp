@@ -351,8 +351,9 @@ struct Document {
 uint32_t le32(const uint8_t *p) { return p[0] | (p[1] << 8) | (p[2] << 16) | (static_cast<uint32_t>(p[3]) << 24); }
 
 // gltf::import: a .glb container (12-B header, JSON chunk, optional BIN chunk) or a JSON
-// document, with buffers from the BIN chunk, data URIs or files beside the document.
-Document load_document(const uint8_t *data, size_t size, const std::string &base_dir) {
+// document, with buffers from the BIN chunk, data URIs or files beside the document (base_dir;
+// nullptr: a document with no directory, whose file URIs are an error).
+Document load_document(const uint8_t *data, size_t size, const std::string *base_dir) {
     Document doc;
     const uint8_t *bin = nullptr;
     size_t bin_size = 0;
@@ -398,8 +399,8 @@ Document load_document(const uint8_t *data, size_t size, const std::string &base
             bytes = base64_decode(uri->str, comma + 1);
         } else {
             if (uri->str.find("://") != std::string::npos) fail("buffer " + std::to_string(i) + ": external URI scheme not supported");
-            std::string path = percent_decode(uri->str);
-            if (!base_dir.empty()) path = base_dir + "/" + path;
+            if (!base_dir) fail("buffer " + std::to_string(i) + ": a file URI needs the document's directory");
+            const std::string path = *base_dir + "/" + percent_decode(uri->str);
             if (!read_file(path, bytes)) fail("cannot read buffer file " + path);
         }
         if (bytes.size() < len) fail("buffer " + std::to_string(i) + " is shorter than its byteLength");
@@ -411,6 +412,9 @@ Document load_document(const uint8_t *data, size_t size, const std::string &base
 // ---- accessors --------------------------------------------------------------------------------
 
 enum : uint32_t { kI8 = 5120, kU8 = 5121, kI16 = 5122, kU16 = 5123, kU32 = 5125, kF32 = 5126 };
+// The most bytes a sparse accessor without a bufferView may take (memory that no buffer of the
+// document backs, so the document's size does not bound it).
+constexpr size_t kMaxSparseOnlyBytes = size_t(1) << 28;
 
 uint32_t component_size(uint32_t t) {
     switch (t) {
@@ -477,13 +481,18 @@ bool read_accessor(const Document &doc, uint64_t index, Accessor &a) {
     const Json *view = j.get("bufferView");
     const Json *sparse = opt_member(j, "sparse", Json::Object, "accessor");
     if (!view && !sparse) return false;
+    // the view's bounds are checked before anything is allocated, so a count in the JSON cannot
+    // ask for more memory than the document's buffers hold; a sparse accessor without a view
+    // (all zeros but its overrides) is capped instead
+    size_t stride = 0;
+    const uint8_t *src = nullptr;
+    if (view)
+        src = view_bytes(doc, as_index(*view, "accessor bufferView"), opt_index(j, "byteOffset", 0, "accessor byteOffset"),
+                         elem, a.count, &stride);
+    else if (a.count * elem > kMaxSparseOnlyBytes)
+        fail("sparse accessor without a bufferView takes more than 256 MiB");
     a.bytes.assign(a.count * elem, 0);
-    if (view) {
-        size_t stride;
-        const uint8_t *src = view_bytes(doc, as_index(*view, "accessor bufferView"),
-                                        opt_index(j, "byteOffset", 0, "accessor byteOffset"), elem, a.count, &stride);
-        for (size_t i = 0; i < a.count; ++i) std::memcpy(a.bytes.data() + i * elem, src + i * stride, elem);
-    }
+    for (size_t i = 0; src && i < a.count; ++i) std::memcpy(a.bytes.data() + i * elem, src + i * stride, elem);
     if (sparse) {
         size_t n = static_cast<size_t>(opt_index(*sparse, "count", 0, "sparse count"));
         const Json *si = opt_member(*sparse, "indices", Json::Object, "sparse");
@@ -698,7 +707,7 @@ void parse(const Document &doc, uint32_t flags, nb_host_scene &scene) {
     }
 }
 
-nb_status parse_into(nb_host_scene *scene, const uint8_t *data, size_t size, const std::string &base_dir, uint32_t flags) {
+nb_status parse_into(nb_host_scene *scene, const uint8_t *data, size_t size, const std::string *base_dir, uint32_t flags) {
     if (flags & ~static_cast<uint32_t>(NB_GLTF_SHARE_GEOMETRY)) return set_error(NB_ERR_INVALID_ARGUMENT, "unknown glTF flags %#x", flags);
     nb_host_scene work = *scene;
     try {
@@ -746,13 +755,15 @@ nb_status nb_host_scene_parse_gltf(nb_host_scene *scene, const char *path, uint3
     std::string dir(path);
     size_t slash = dir.find_last_of('/');
     dir = slash == std::string::npos ? std::string(".") : dir.substr(0, slash == 0 ? 1 : slash);
-    return parse_into(scene, bytes.data(), bytes.size(), dir, flags);
+    return parse_into(scene, bytes.data(), bytes.size(), &dir, flags);
 }
 
 nb_status nb_host_scene_parse_gltf_memory(nb_host_scene *scene, const void *data, size_t size, const char *base_dir,
                                           uint32_t flags) {
     if (!scene || (!data && size)) return set_error(NB_ERR_INVALID_ARGUMENT, "null scene or data");
-    return parse_into(scene, static_cast<const uint8_t *>(data), size, base_dir ? base_dir : "", flags);
+    if (!base_dir) return parse_into(scene, static_cast<const uint8_t *>(data), size, nullptr, flags);
+    const std::string dir(base_dir);
+    return parse_into(scene, static_cast<const uint8_t *>(data), size, &dir, flags);
 }
 
 nb_status nb_host_scene_counts(const nb_host_scene *scene, uint32_t *n_vertices, uint32_t *n_indices, uint32_t *n_prims) {

@@ -217,7 +217,9 @@ def _one_triangle_doc():
     return _split(gltf.glb_bytes(nb.Scene(v, np.array([0, 1, 2, 2, 1, 3], dtype=np.uint32), p)))
 
 
-def test_interleaved_sparse_and_normalized_accessors(tmp_path):
+def interleaved_sparse_doc():
+    """The one-triangle document with POSITION and NORMAL interleaved in one strided view,
+    TEXCOORD_0 as normalized u16 and a sparse override of tangent 2."""
     doc, binary = _one_triangle_doc()
     # interleave POSITION and NORMAL into one strided view
     acc = doc["accessors"]
@@ -248,7 +250,11 @@ def test_interleaved_sparse_and_normalized_accessors(tmp_path):
     acc[3]["sparse"] = {"count": 1, "indices": {"bufferView": nv - 2, "componentType": 5121},
                         "values": {"bufferView": nv - 1}}
     doc["buffers"][0]["byteLength"] = len(binary)
-    data = _join(doc, binary)
+    return doc, binary
+
+
+def test_interleaved_sparse_and_normalized_accessors(tmp_path):
+    data = _join(*interleaved_sparse_doc())
     s = _parse_bytes(data, tmp_path)
     _same(s, og.parse_gltf(data))
     assert np.array_equal(s.vertices["tangent"][2], [0, 1, 0, -1])
@@ -267,6 +273,46 @@ def test_gltf_json_with_data_uri_and_external_buffer(tmp_path):
     (tmp_path / "tri data.bin").write_bytes(bytes(binary))
     (tmp_path / "b.gltf").write_text(json.dumps(doc))
     _same(nb.Scene().parse_gltf(tmp_path / "b.gltf"), (want.vertices, want.indices, want.primitives))
+
+
+def test_memory_parse_reads_files_only_beside_a_given_directory(tmp_path, monkeypatch):
+    """nb_host_scene_parse_gltf_memory with no base directory refuses a file URI (even one that
+    names a file in the working directory); with the directory it reads the file beside it."""
+    import ctypes
+    from nbounce_amd._lib import lib
+    doc, binary = _one_triangle_doc()
+    (tmp_path / "tri.bin").write_bytes(bytes(binary))
+    doc["buffers"][0]["uri"] = "tri.bin"
+    text = json.dumps(doc).encode()
+    monkeypatch.chdir(tmp_path)
+    h = ctypes.c_void_p()
+    assert lib.nb_host_scene_create(ctypes.byref(h)) == 0
+    try:
+        assert lib.nb_host_scene_parse_gltf_memory(h, text, len(text), None, 0) == 7   # NB_ERR_FORMAT
+        assert b"needs the document's directory" in lib.nb_last_error()
+        assert lib.nb_host_scene_parse_gltf_memory(h, text, len(text), str(tmp_path).encode(), 0) == 0
+        nv, ni, npr = ctypes.c_uint32(), ctypes.c_uint32(), ctypes.c_uint32()
+        assert lib.nb_host_scene_counts(h, ctypes.byref(nv), ctypes.byref(ni), ctypes.byref(npr)) == 0
+        assert (nv.value, ni.value, npr.value) == (4, 6, 1)
+    finally:
+        lib.nb_host_scene_destroy(h)
+
+
+def test_sparse_accessor_without_a_view_is_capped(tmp_path):
+    """A sparse accessor with no bufferView costs memory that no buffer backs: one of more than
+    256 MiB is refused before anything is allocated; one within the cap reads as zeros plus its
+    overrides (here the whole TANGENT accessor)."""
+    doc, binary = interleaved_sparse_doc()
+    acc = doc["accessors"][3]
+    del acc["bufferView"]
+    s = _parse_bytes(_join(doc, binary), tmp_path)
+    want = np.zeros((4, 4), np.float32)
+    want[2] = [0, 1, 0, -1]
+    assert np.array_equal(s.vertices["tangent"], want)
+    acc["count"] = (1 << 24) + 1          # VEC4 f32: 16 B each, just over 256 MiB
+    with pytest.raises(nb.NbError) as e:
+        _parse_bytes(_join(doc, binary), tmp_path)
+    assert e.value.status == 7 and "256 MiB" in str(e.value)
 
 
 def test_default_and_emissive_materials(tmp_path):

@@ -1,0 +1,95 @@
+"""Mutation fuzzing of the readers of untrusted files: the glTF / GLB reader (gltf.cpp, which
+replaces Scene::parse_gltf, scene.rs:108-217) and the cube DDS reader (dds.cpp, which replaces
+EnvMap::load, envmap.rs:12 and texture.rs:89-141).  tests/cpp/fuzz_parsers is built from the
+readers' own sources under AddressSanitizer and UndefinedBehaviorSanitizer; it mutates each seed
+deterministically and fails on any sanitizer report (out-of-bounds access, use after free,
+undefined behaviour, a leak, an allocation above 1 GiB), on a scene that parses without geometry
+sharing but not with it, on primitive ranges outside the index array, and on a DDS decode that
+disagrees with its own size query or writes past the cube.
+
+Seeds: the reference's spheres.glb and testscene.glb; a one-triangle GLB; the same document as
+.gltf JSON with a base64 data-URI buffer; a document with interleaved, normalized and sparse
+accessors and a TRS node; 8x8 cubes written as RGBA32F, RGBA16F and BC6H_UF16 (DX10 headers)
+and as RGBA32F with the legacy D3DFMT header.  The counts of mutants that parse show that the
+mutations reach past the container checks into the accessor, material and block-decoding code."""
+import base64
+import json
+import os
+import re
+import struct
+import subprocess
+
+import numpy as np
+import pytest
+
+import nbounce_amd as nb
+from nbounce_amd import pathtracer
+from tests import test_gltf as tg
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+EXE = os.path.join(ROOT, "tests", "cpp", "fuzz_parsers")
+GOLDEN = os.path.join(ROOT, "tests", "golden")
+
+
+def _run(iterations, seed, paths):
+    if not os.path.exists(EXE):
+        pytest.fail("tests/cpp/fuzz_parsers is not built: run __graft_entry__.build()")
+    r = subprocess.run([EXE, str(iterations), str(seed)] + [str(p) for p in paths],
+                       capture_output=True, text=True, timeout=600)
+    print(r.stdout[-4000:], r.stderr[-8000:])
+    assert r.returncode == 0 and "FAIL" not in r.stdout, "fuzzer failed (the sanitizer report is above)"
+    counts = {}
+    for line in r.stdout.splitlines():
+        m = re.match(r"(.*): (\d+) mutants, (\d+) parsed", line)
+        if m:
+            counts[os.path.basename(m.group(1))] = (int(m.group(2)), int(m.group(3)))
+    assert len(counts) == len(paths)
+    return counts
+
+
+def _gltf_seeds(tmp_path):
+    doc, binary = tg._one_triangle_doc()
+    (tmp_path / "tri.glb").write_bytes(tg._join(doc, binary))
+    doc["buffers"][0]["uri"] = "data:application/octet-stream;base64," + base64.b64encode(bytes(binary)).decode()
+    (tmp_path / "tri.gltf").write_text(json.dumps(doc))
+    doc, binary = tg.interleaved_sparse_doc()
+    node = doc["nodes"][0]
+    del node["matrix"]
+    node.update({"translation": [1.5, -2, 0.25], "rotation": [0.2, -0.4, 0.1, 0.9539392], "scale": [2, 0.5, 1.25]})
+    (tmp_path / "sparse.glb").write_bytes(tg._join(doc, binary))
+    return [tmp_path / "tri.glb", tmp_path / "tri.gltf", tmp_path / "sparse.glb"]
+
+
+def _dds_seeds(tmp_path):
+    faces = (np.random.default_rng(5).random((6, 8, 8, 4)) * 4).astype(np.float32)
+    out = []
+    for name, fmt in (("rgba32f.dds", pathtracer.DDS_RGBA32F), ("rgba16f.dds", pathtracer.DDS_RGBA16F),
+                      ("bc6h.dds", pathtracer.DDS_BC6H_UF16)):
+        pathtracer.write_dds_cube(tmp_path / name, faces, fmt)
+        out.append(tmp_path / name)
+    # the legacy header: FourCC 116 (D3DFMT_A32B32G32R32F) and no DX10 extension
+    d = bytearray((tmp_path / "rgba32f.dds").read_bytes())
+    struct.pack_into("<I", d, 84, 116)
+    legacy = bytes(d[:128] + d[148:])
+    (tmp_path / "legacy.dds").write_bytes(legacy)
+    assert np.array_equal(nb.read_dds_cube(tmp_path / "legacy.dds"), nb.read_dds_cube(tmp_path / "rgba32f.dds"))
+    out.append(tmp_path / "legacy.dds")
+    return out
+
+
+def test_gltf_reader_survives_mutants(tmp_path):
+    counts = _run(4000, 1, _gltf_seeds(tmp_path))
+    for name, (n, ok) in counts.items():
+        assert 0.05 * n < ok < 0.9 * n, (name, ok)
+
+
+def test_reference_assets_survive_mutants(tmp_path):
+    counts = _run(300, 2, [os.path.join(GOLDEN, "spheres.glb"), os.path.join(GOLDEN, "testscene.glb")])
+    for name, (n, ok) in counts.items():
+        assert 0.05 * n < ok < 0.9 * n, (name, ok)
+
+
+def test_dds_reader_survives_mutants(tmp_path):
+    counts = _run(3000, 3, _dds_seeds(tmp_path))
+    for name, (n, ok) in counts.items():
+        assert 0.1 * n < ok < 0.9 * n, (name, ok)
