@@ -103,7 +103,7 @@ nb_status check_tree(const nb_bvh_node *nodes, uint32_t n, uint32_t n_prims, con
             if (d.start > kLeafStartMask)
                 return set_error(NB_ERR_UNSUPPORTED, "%s node %u: leaf start %u exceeds 2^26", name, i, d.start);
         } else {
-            if (d.start <= i || d.start + 1 >= n)
+            if (d.start <= i || d.start >= n - 1)   // (d.start + 1 would wrap for start = 2^32 - 1)
                 return set_error(NB_ERR_BAD_LAYOUT, "%s node %u: children %u, %u not after it", name, i, d.start, d.start + 1);
             uint8_t cd = static_cast<uint8_t>(depth[i] + 1);
             if (cd > kMaxTreeDepth)

@@ -193,3 +193,71 @@ def test_render_arguments_are_validated(nb, ctx, small_gpu):
     fresh.dispatch()
     fresh.dispatch()
     np.testing.assert_array_equal(pt.output_texture().view(np.uint32), fresh.output_texture().view(np.uint32))
+
+
+def _chain_blas(depth):
+    """A BLAS whose left spine is `depth` inner nodes deep: inner node k has children (2k+1, 2k+2),
+    the left one the next inner node, the right one a leaf; the last inner node has two leaves.
+    Every leaf holds triangle 0."""
+    from nbounce_amd import layout
+    n = 2 * depth + 1
+    nodes = np.zeros(n, dtype=layout.NODE)
+    nodes["min"], nodes["max"] = (-1.0, -1.0, -1.0), (1.0, 1.0, 1.0)
+    for k in range(depth):
+        i = 0 if k == 0 else 2 * k - 1
+        nodes[i]["start"], nodes[i]["end"] = 2 * k + 1, 0
+        nodes[2 * k + 2]["start"], nodes[2 * k + 2]["end"] = 0, 1
+    nodes[2 * depth - 1]["start"], nodes[2 * depth - 1]["end"] = 0, 1
+    return nodes
+
+
+def test_malformed_scene_layouts_are_rejected(nb, ctx, small_gpu):
+    """nb_scene_create validates the caller's reference-layout arrays on the host before anything
+    reaches the device, since the kernels follow the node references they are given: an index
+    past the vertices, an inner node whose children are not after it (including a child index of
+    2^32 - 1, where the second child's index wraps to 0), a second child past the array, a leaf
+    range outside the primitives or empty, a tree deeper than 32 levels, an instance whose BLAS
+    root is out of range and a TLAS without instances each fail with NB_ERR_BAD_LAYOUT.  A tree
+    exactly 32 levels deep is accepted, and a render on the context afterwards is unaffected."""
+    from nbounce_amd import scenegen
+    w, sb, env = small_gpu
+    blas, tlas, inst, v, idx = w["blas"], w["tlas"], w["instances"], w["vertices"], w["indices"]
+    inner_b = int(np.nonzero(blas["end"] == 0)[0][0])
+    leaf_t = int(np.nonzero(tlas["end"] > 0)[0][0])
+
+    def rejected(b=blas, t=tlas, i=inst, vv=v, ix=idx, what=""):
+        print("case: malformed layout:", what, flush=True)
+        with pytest.raises(nb.NbError) as e:
+            nb.SceneBuffers(ctx, b, t, i, vv, ix)
+        assert e.value.status == 4, (what, str(e.value))   # NB_ERR_BAD_LAYOUT
+
+    ix = idx.copy(); ix[5] = len(v)
+    rejected(ix=ix, what="index past the vertices")
+    b = blas.copy(); b[inner_b]["start"] = inner_b
+    rejected(b=b, what="children not after their parent")
+    b = blas.copy(); b[inner_b]["start"] = 0xFFFFFFFF
+    rejected(b=b, what="child index 2^32 - 1")
+    b = blas.copy(); b[inner_b]["start"] = len(blas) - 1
+    rejected(b=b, what="second child past the array")
+    leaf_b = int(np.nonzero(blas["end"] > 0)[0][0])
+    b = blas.copy(); b[leaf_b]["end"] = len(idx) // 3 + 1
+    rejected(b=b, what="leaf range past the triangles")
+    b = blas.copy(); b[leaf_b]["start"] = b[leaf_b]["end"]
+    rejected(b=b, what="empty leaf range")
+    t = tlas.copy(); t[leaf_t]["end"] = len(inst) + 1
+    rejected(t=t, what="TLAS leaf range past the instances")
+    i2 = inst.copy(); i2[3]["node"] = len(blas)
+    rejected(i=i2, what="instance BLAS root out of range")
+    rejected(i=inst[:0], what="TLAS without instances")
+    # a one-instance scene over a chain BLAS: 33 inner levels are refused, 32 accepted
+    i1 = inst[:1].copy(); i1[0]["node"] = 0
+    t1 = np.zeros(1, dtype=tlas.dtype); t1[0]["min"], t1[0]["max"] = (-1e3,) * 3, (1e3,) * 3
+    t1[0]["start"], t1[0]["end"] = 0, 1
+    rejected(b=_chain_blas(33), t=t1, i=i1, what="BLAS 33 levels deep")
+    print("case: BLAS exactly 32 levels deep is accepted", flush=True)
+    nb.SceneBuffers(ctx, _chain_blas(32), t1, i1, v, idx).close()
+    # the context still renders the valid scene exactly as the oracle does
+    print("case: valid render after the rejections", flush=True)
+    got, ref = _frames_vs_oracle(nb, ctx, sb, W.oracle_scene(w), env, (32, 32), scenegen.SMALL["eye"],
+                                 scenegen.SMALL["target"], 2)
+    _check(got, ref)
