@@ -7,7 +7,9 @@
 // Every seed file is mutated `iterations` times, deterministically from `seed`.  Each mutant must
 // come back as a status; a glTF mutant that parses must leave a scene whose arrays copy out,
 // whose primitives' index ranges lie inside its index array and which parses with geometry
-// sharing too, and a DDS mutant that decodes must fill exactly 6 x n x n RGBA texels.  No mutant
+// sharing too, and a DDS mutant that decodes must fill exactly 6 x n x n RGBA texels.  A parsed
+// scene then goes through the host scene builder (scene_build.cpp), whose trees must obey the
+// rules the scene upload checks.  No mutant
 // may make a reader ask for more than 1 GiB of memory.  (Index
 // values are not checked against the vertex count by the reader, as in the reference; the scene
 // builder rejects out-of-range ones.)  GLB seeds are mutated as a JSON chunk plus a BIN chunk and put back
@@ -18,7 +20,9 @@
 // usage: fuzz_parsers <iterations> <seed> <file.glb | file.gltf | file.dds>...
 // Prints one line per seed: the mutants tried, how many parsed, and the statuses of the rest.
 #include "nbounce.h"
+#include "nb_internal.h"
 
+#include <algorithm>
 #include <cstdarg>
 #include <cstdint>
 #include <cstdio>
@@ -185,6 +189,59 @@ bool ends_with(const std::string &s, const char *suffix) {
     return s.size() >= n && s.compare(s.size() - n, n, suffix) == 0;
 }
 
+[[noreturn]] void fail_build(const char *what, uint32_t a, uint32_t b) {
+    std::printf("FAIL: scene build: %s (%u, %u)\n", what, a, b);
+    std::exit(1);
+}
+
+// The tree rules nb_scene_create enforces: children after their parent and inside the array,
+// leaf ranges non-empty inside [0, n_prims), at most 32 levels.
+void check_built_tree(const std::vector<nb_bvh_node> &t, uint32_t n, uint32_t n_prims) {
+    std::vector<uint32_t> depth(n, 0);
+    for (uint32_t i = 0; i < n; ++i) {
+        const nb_bvh_node &d = t[i];
+        if (d.end > 0) {
+            if (d.start >= d.end || d.end > n_prims) fail_build("leaf range outside the primitives", i, d.end);
+        } else {
+            if (d.start <= i || d.start >= n - 1) fail_build("children not after their parent", i, d.start);
+            if (depth[i] + 1 > 32) fail_build("tree deeper than 32 levels", i, depth[i]);
+            depth[d.start] = std::max(depth[d.start], depth[i] + 1);
+            depth[d.start + 1] = std::max(depth[d.start + 1], depth[i] + 1);
+        }
+    }
+}
+
+// A parsed scene through the host builder (scene_build.cpp, SceneBuffers::from_scene's build),
+// with and without BLAS sharing: it must return a status, and a built scene must obey the tree
+// rules, point its instances at BLAS nodes and keep the index values (a permutation).
+void check_build(const std::vector<nb_vertex> &v, const std::vector<uint32_t> &idx, const std::vector<nb_primitive> &p) {
+    if (p.empty()) return;
+    for (uint32_t flags : {0u, static_cast<uint32_t>(NB_BUILD_SHARE_BLAS)}) {
+        uint32_t bcap = 0, tcap = 0;
+        if (nb::build_scene_capacity(static_cast<uint32_t>(idx.size()), p.data(), static_cast<uint32_t>(p.size()), flags,
+                                     &bcap, &tcap) != NB_OK)
+            continue;
+        if (bcap > (1u << 15)) continue;   // large scenes: their builds are tested elsewhere, and slow here
+        std::vector<uint32_t> ix(idx);
+        std::vector<nb_bvh_node> blas(bcap), tlas(tcap);
+        std::vector<nb_instance> inst(p.size());
+        uint32_t nb_ = 0, nt = 0;
+        if (nb::build_scene(v.data(), static_cast<uint32_t>(v.size()), ix.data(), static_cast<uint32_t>(ix.size()), p.data(),
+                            static_cast<uint32_t>(p.size()), flags, blas.data(), bcap, &nb_, tlas.data(), tcap, &nt,
+                            inst.data()) != NB_OK)
+            continue;
+        if (nb_ > bcap || nt > tcap) fail_build("more nodes than the capacity", nb_, nt);
+        check_built_tree(blas, nb_, static_cast<uint32_t>(ix.size() / 3));
+        check_built_tree(tlas, nt, static_cast<uint32_t>(p.size()));
+        for (const nb_instance &q : inst)
+            if (q.node >= nb_) fail_build("instance BLAS root outside the BLAS", q.node, nb_);
+        std::vector<uint32_t> a(idx), b(ix);
+        std::sort(a.begin(), a.end());
+        std::sort(b.begin(), b.end());
+        if (a != b) fail_build("index values changed", 0, 0);
+    }
+}
+
 // One glTF mutant through the reader; returns its status.  A parse that succeeds is checked.
 nb_status try_gltf(const Bytes &data) {
     nb_host_scene *sc = nullptr;
@@ -202,6 +259,7 @@ nb_status try_gltf(const Bytes &data) {
                 std::printf("FAIL: primitive range [%u, %u) outside %u indices\n", q.index_start, q.index_end, ni);
                 std::exit(1);
             }
+        check_build(v, idx, p);
         // the same document with geometry sharing must parse too
         nb_host_scene *sh = nullptr;
         if (nb_host_scene_create(&sh) != NB_OK) std::abort();

@@ -4,13 +4,14 @@ EnvMap::load, envmap.rs:12 and texture.rs:89-141).  tests/cpp/fuzz_parsers is bu
 readers' own sources under AddressSanitizer and UndefinedBehaviorSanitizer; it mutates each seed
 deterministically and fails on any sanitizer report (out-of-bounds access, use after free,
 undefined behaviour, a leak, an allocation above 1 GiB), on a scene that parses without geometry
-sharing but not with it, on primitive ranges outside the index array, and on a DDS decode that
-disagrees with its own size query or writes past the cube.
+sharing but not with it, on primitive ranges outside the index array, on a host scene build
+(scene_build.cpp) of a parsed scene whose trees break the rules the scene upload checks, and on
+a DDS decode that disagrees with its own size query or writes past the cube.
 
 Seeds: the reference's spheres.glb and testscene.glb; a one-triangle GLB; the same document as
 .gltf JSON with a base64 data-URI buffer; a document with interleaved, normalized and sparse
-accessors and a TRS node; 8x8 cubes written as RGBA32F, RGBA16F and BC6H_UF16 (DX10 headers)
-and as RGBA32F with the legacy D3DFMT header.  The counts of mutants that parse show that the
+accessors and a TRS node; three spheres in a box (about 250 triangles); 8x8 cubes written as
+RGBA32F, RGBA16F and BC6H_UF16 (DX10 headers) and as RGBA32F with the legacy D3DFMT header.  The counts of mutants that parse show that the
 mutations reach past the container checks into the accessor, material and block-decoding code."""
 import base64
 import json
@@ -23,7 +24,7 @@ import numpy as np
 import pytest
 
 import nbounce_amd as nb
-from nbounce_amd import pathtracer
+from nbounce_amd import gltf, pathtracer, scenegen
 from tests import test_gltf as tg
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
@@ -57,7 +58,10 @@ def _gltf_seeds(tmp_path):
     del node["matrix"]
     node.update({"translation": [1.5, -2, 0.25], "rotation": [0.2, -0.4, 0.1, 0.9539392], "scale": [2, 0.5, 1.25]})
     (tmp_path / "sparse.glb").write_bytes(tg._join(doc, binary))
-    return [tmp_path / "tri.glb", tmp_path / "tri.gltf", tmp_path / "sparse.glb"]
+    # three spheres in a box (about 250 triangles in 4 primitives): builds that use binned splits
+    v, i, pr = scenegen.spheres_in_box(3, n_spheres=3, n_emissive=1, subdivisions=1)
+    (tmp_path / "boxed.glb").write_bytes(gltf.glb_bytes(nb.Scene(v, i, pr)))
+    return [tmp_path / "tri.glb", tmp_path / "tri.gltf", tmp_path / "sparse.glb", tmp_path / "boxed.glb"]
 
 
 def _dds_seeds(tmp_path):
