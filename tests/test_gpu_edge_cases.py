@@ -261,3 +261,36 @@ def test_malformed_scene_layouts_are_rejected(nb, ctx, small_gpu):
     got, ref = _frames_vs_oracle(nb, ctx, sb, W.oracle_scene(w), env, (32, 32), scenegen.SMALL["eye"],
                                  scenegen.SMALL["target"], 2)
     _check(got, ref)
+
+
+def test_non_finite_inputs_render_without_fault(nb, ctx, small_gpu):
+    """Non-finite values reach the kernels as data, not as addresses: the slab test misses a NaN
+    box or ray, the triangle test rejects a NaN triangle, and the environment lookup clamps a NaN
+    direction to its border texels.  A camera of NaNs renders (every pixel NaN, alpha 1); a scene
+    whose diffuse triangle has NaN vertices renders as the oracle renders it (that triangle is
+    never hit); a render of the valid scene afterwards is unaffected."""
+    from nbounce_amd import scenegen
+    w, sb, env = small_gpu
+    print("case: NaN camera, 32x32, 2 frames", flush=True)
+    cam = nb.CameraController(world_position=scenegen.SMALL["eye"], target=scenegen.SMALL["target"], aspect_ratio=1.0)
+    pt = nb.Pathtracer(ctx, sb, cam, env, 32, 32, resolution_factor=1.0, table=W.table(), sobol_stride=17)
+    pt._camera_data = pt._camera_data.copy()
+    pt._camera_data.view(np.float32)[:] = np.nan
+    pt.dispatch()
+    pt.dispatch()
+    img = pt.output_texture()
+    assert np.isnan(img[..., :3]).all() and (img[..., 3] == 1.0).all()
+    print("case: a triangle of NaN vertices, 32x32, 4 frames", flush=True)
+    v, i, pr = _two_triangles()
+    v["position"][3:6] = np.nan
+    blas, tlas, inst, idx = oracle.build_scene(v, i, pr)
+    sb2 = nb.SceneBuffers(ctx, blas, tlas, inst, v, idx)
+    env2 = nb.EnvMap(ctx, scenegen.sky_cube(32))
+    got, ref = _frames_vs_oracle(nb, ctx, sb2, oracle.Scene(blas, tlas, inst, v, idx), env2, (32, 32),
+                                 (0.0, 0.0, 4.0), (0.0, 0.0, 0.0), 4)
+    _check(got, ref)
+    sb2.close()
+    print("case: valid render after the non-finite inputs", flush=True)
+    got, ref = _frames_vs_oracle(nb, ctx, sb, W.oracle_scene(w), env, (32, 32), scenegen.SMALL["eye"],
+                                 scenegen.SMALL["target"], 2)
+    _check(got, ref)
