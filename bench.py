@@ -512,6 +512,17 @@ def cache_bench(args):
     print(json.dumps(out), flush=True)
 
 
+def kernel_name(frames_per_launch, pixels):
+    """The kernel a launch of the timed region runs: nb_render_frames of two or more frames on a
+    target of at most 2^19 pixels renders one path per (frame, pixel) with the serving kernel
+    (unless NB_SAMPLE_PARALLEL=0 or NB_SAMPLE_PARALLEL_PIXELS lowers the limit)."""
+    if frames_per_launch >= 2 and os.environ.get("NB_SAMPLE_PARALLEL", "1") != "0":
+        limit = int(os.environ.get("NB_SAMPLE_PARALLEL_PIXELS", str(1 << 19)))
+        if pixels <= limit:
+            return "serve_pt_kernel<16, 4> (sample-parallel frames)"
+    return "render_pt_kernel" if os.environ.get("NB_RENDER", "pt") != "tile" else "render_kernel"
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -702,7 +713,7 @@ def main():
             "traffic_source": "rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE (L2 to memory, Infinity Cache hits "
                               "included: an upper bound on HBM bytes), scaled by a 256 MiB copy "
                               "(profiles/render_pt_kernel_instanced_1080p_traffic.md)" if traffic else None,
-            "kernel": "render_pt_kernel" if os.environ.get("NB_RENDER", "pt") != "tile" else "render_kernel",
+            "kernel": kernel_name(F, W * H),
             "kernel_ms": round(kernel_ms, 4),
             "kernel_ms_is": ("the timed region / launches: pipelined path kernels of consecutive frames overlap, "
                              "so each launch's own duration is longer; the mix kernels are inside")
