@@ -713,7 +713,7 @@ def main():
             "traffic_source": "rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE (L2 to memory, Infinity Cache hits "
                               "included: an upper bound on HBM bytes), scaled by a 256 MiB copy "
                               "(profiles/render_pt_kernel_instanced_1080p_traffic.md)" if traffic else None,
-            "kernel": kernel_name(F, W * H),
+            "kernel": kernel_name(F, P["width"] * P["height"]),
             "kernel_ms": round(kernel_ms, 4),
             "kernel_ms_is": ("the timed region / launches: pipelined path kernels of consecutive frames overlap, "
                              "so each launch's own duration is longer; the mix kernels are inside")
