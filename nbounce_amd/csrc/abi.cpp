@@ -547,7 +547,11 @@ nb_status render_frames_sample_parallel(nb_context *ctx, const nb_scene *scene, 
         st.counters = ctx->counting ? ctx->counters : nullptr;
         st.regen_min_idle = 24;
         st.inner_quorum = 28;
-        st.grab_units = 4;
+        // 2-unit grabs (32 paths): a 256x256, 16-frame launch holds about 4 paths per lane, and
+        // a wave's last grab sets its tail (Mrays/s: 1 unit 2157, 2 units 2222-2227, 4 units
+        // 2182-2212, 8 units 2045-2082; refills after 16 or 32 waiting lanes instead of 24: 2155,
+        // 2134)
+        st.grab_units = 2;
         NB_HIP(hipMemcpyAsync(d, &h, sizeof(SpDesc), hipMemcpyHostToDevice, ctx->stream));
         NB_TRY(serve_launch_reset(&d->step, ctx->stream));
         NB_TRY(serve_launch_step(&d->step, ctx->stream, 4u));
