@@ -1,9 +1,10 @@
-"""Static checks of the scripts that run only on a GPU (bench.py, __graft_entry__.py), so that a
-name error in a branch the CPU suite cannot execute is caught here: every name a function reads
-must be a parameter, a name it binds, a name an enclosing function binds, a module-level name
-or a builtin."""
+"""Static checks of the Python sources, above all the scripts that run only on a GPU (bench.py,
+__graft_entry__.py, the GPU tests), so that a name error in a branch the CPU suite cannot execute
+is caught here: every name a function reads must be a parameter, a name it binds, a name an
+enclosing function binds, a module-level name or a builtin."""
 import ast
 import builtins
+import glob
 import os
 
 import pytest
@@ -67,8 +68,13 @@ def _undefined(path):
     return missing
 
 
-@pytest.mark.parametrize("name", ["bench.py", "__graft_entry__.py"])
-def test_gpu_scripts_read_only_defined_names(name):
+SOURCES = sorted(["bench.py", "__graft_entry__.py"] +
+                 [os.path.relpath(p, ROOT) for d in ("nbounce_amd", "oracle", "tests", "tests/golden")
+                  for p in glob.glob(os.path.join(ROOT, d, "*.py"))])
+
+
+@pytest.mark.parametrize("name", SOURCES)
+def test_sources_read_only_defined_names(name):
     missing = _undefined(os.path.join(ROOT, name))
     assert not missing, missing
 
