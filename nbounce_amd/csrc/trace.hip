@@ -378,6 +378,8 @@ static uint32_t regen_grid() {
 
 // NB_INNER_QUORUM=n (0..64): the traversal phase runs inner-node steps alone while more than n
 // lanes have one (64: never).  28 measured best on the instanced 1080p workload (24..32 within 1%).
+// With the refitted TLAS, pipelined: 28 2232-2237 Mrays/s, 24 2209-2210, 32 2234-2241 (within
+// the noise of 28), 36 2210-2212, 40 2141-2143.
 static uint32_t inner_quorum() {
     static const uint32_t v = [] {
         const char *e = std::getenv("NB_INNER_QUORUM");
